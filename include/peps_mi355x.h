@@ -1,0 +1,181 @@
+/*
+ * peps_mi355x.h -- C ABI of libpeps_mi355x.so, the MI355X (gfx950) explicit-C
+ * PEPS row-boundary contraction that counts N-Queens solutions.
+ *
+ * Every entry point is plain C: pointers, sizes and POD structs, no C++ or
+ * torch types.  Functions return 0 on success; on failure they return a
+ * nonzero PEPS_E_* code and copy a NUL-terminated message into `err`
+ * (truncated to `err_cap` bytes; `err` may be NULL).  The caller owns every
+ * buffer.  Results are exact: coefficients are carried as residues modulo the
+ * reference's 32-bit CRT primes and reconstructed on the host into a checked
+ * 128-bit integer.
+ *
+ * Reference interface each entry point replaces (reference crate
+ * `nqueens-peps-naive`, paths relative to its root):
+ *   peps_contract_rows            <- pub fn contract_rows            src/lib.rs:757-760
+ *                                    (= contract_rows_d4_orbit_sort_reduce src/lib.rs:784-786,
+ *                                     body contract_rows_sort_reduce_with_modes src/lib.rs:5287-5415)
+ *   peps_contract_rows_with_modes <- contract_rows_sort_reduce      src/lib.rs:770-772 (symmetry None)
+ *                                    contract_rows_d4_orbit_sort_reduce src/lib.rs:784-786 (orbits)
+ *   peps_known_count              <- pub fn known_count             src/lib.rs:5752-5784
+ *   peps_peak_rss_bytes           <- pub fn peak_rss_bytes          src/lib.rs:5786-5850
+ *   peps_crt_plan                 <- pub fn wide_crt_plan           src/lib.rs:3859-3886
+ *   peps_crt_reconstruct          <- fn reconstruct_crt             src/lib.rs:4134-4163
+ *   peps_prefix_tasks             <- fn build_wide_crt_tasks        src/lib.rs:3888-3956
+ *   peps_count_prefix             <- (no single reference function) the row contraction
+ *                                    started from the boundary left by k placed rows;
+ *                                    pinned by the DFS comparator src/dfs_bitmask.rs:67-109
+ *   peps_count_batched            <- sliced form of contract_rows over the prefix tasks of
+ *                                    build_wide_crt_tasks src/lib.rs:3888-3956
+ *   peps_server_*                 <- (no counterpart) serving loop over peps_count_prefix
+ */
+#ifndef PEPS_MI355X_H
+#define PEPS_MI355X_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Largest board the reference's packed u128 boundary accepts (src/lib.rs:5292-5294). */
+#define PEPS_MAX_N 42
+/* Largest board whose 3N-bit boundary key fits the device's u64 key. */
+#define PEPS_MAX_DEVICE_N 21
+/* Largest number of CRT lanes (reference CRT_PRIME_CANDIDATES, src/lib.rs:3767). */
+#define PEPS_MAX_LANES 4
+
+enum {
+  PEPS_OK = 0,
+  PEPS_E_INVALID = 1,      /* bad argument (N too large, bad options, ...) */
+  PEPS_E_OVERFLOW = 2,     /* exactness bound violated (CRT > N!, u128 overflow) */
+  PEPS_E_DEVICE = 3,       /* HIP runtime error or no device */
+  PEPS_E_MEMORY = 4,       /* HBM budget exceeded */
+  PEPS_E_UNKNOWN = 5,      /* peps_known_count: N outside the table */
+  PEPS_E_BUSY = 6          /* serving: no free slot / queue full */
+};
+
+/* Rust u128 as two little-endian 64-bit halves. */
+typedef struct {
+  uint64_t lo;
+  uint64_t hi;
+} peps_u128;
+
+/* Mirrors LayerMetric (src/lib.rs:388-401) plus device extras. */
+typedef struct {
+  uint32_t row; /* 0-based row index, as LayerMetric.row */
+  uint64_t input_states;
+  peps_u128 tensor_entries_examined;
+  peps_u128 tensor_entries_matched;
+  peps_u128 row_operator_candidates;
+  peps_u128 row_operator_matched;
+  peps_u128 completed_row_terms;
+  uint64_t output_states;
+  peps_u128 output_weight;
+  double elapsed_s;
+  uint64_t peak_rss_bytes;
+  /* device extras (0 on host-only paths) */
+  uint64_t device_bytes;      /* device arena bytes live during this row */
+  uint32_t partition_levels;  /* hash-radix partition levels used for the merge */
+  uint32_t final_buckets;     /* LDS merge buckets used for the merge */
+} peps_layer_metric;
+
+/* Mirrors ContractionResult (src/lib.rs:403-415) plus device extras. */
+typedef struct {
+  uint32_t n;
+  peps_u128 count;
+  double elapsed_s;
+  uint64_t peak_states;
+  peps_u128 tensor_entries_examined;
+  peps_u128 tensor_entries_matched;
+  peps_u128 row_operator_candidates;
+  peps_u128 row_operator_matched;
+  uint64_t peak_rss_bytes;
+  uint32_t layer_count;
+  peps_layer_metric layers[PEPS_MAX_N];
+  /* device extras */
+  uint32_t crt_lanes;
+  uint64_t primes[PEPS_MAX_LANES];
+  uint64_t residues[PEPS_MAX_LANES];
+  uint64_t device_peak_bytes;
+  double kernel_s; /* device time between the first and last kernel of the solve */
+} peps_contraction_result;
+
+/* Symmetry mode (reference SymmetryMode, src/lib.rs:715-719). */
+enum { PEPS_SYMMETRY_NONE = 0, PEPS_SYMMETRY_TOP_ROW_ORBITS = 1 };
+
+/* Tuning and resource options.  Zero-initialised means "defaults". */
+typedef struct {
+  uint32_t crt_lanes;        /* 0: reference plan (shortest prime prefix with M > N!); else 1..4 */
+  uint32_t radix_bits;       /* hash-radix digit width per partition level (0: auto; 4..11) */
+  uint32_t tile_states;      /* target records per LDS merge bucket (0: auto; 256..4096) */
+  uint32_t device;           /* HIP device ordinal */
+  uint64_t hbm_budget_bytes; /* arena cap (0: auto, 85% of free HBM) */
+  uint32_t collect_layers;   /* nonzero: fill per-layer metrics (needs a host sync per row) */
+  uint32_t reserved[7];
+} peps_options;
+
+/* ---------------------------------------------------------------- solve -- */
+
+/* Reference contract_rows(n): first-row vertical-reflection orbit slicing, one
+ * merged frontier, exact count.  N = 0 returns count 1 (src/lib.rs:5295-5307).
+ * N > 42 fails with the reference text; 21 < N <= 42 fails with the device's
+ * u64 key limit. */
+int peps_contract_rows(uint32_t n, const peps_options* opts, peps_contraction_result* out,
+                       char* err, size_t err_cap);
+
+/* Same with an explicit symmetry mode (PEPS_SYMMETRY_*). */
+int peps_contract_rows_with_modes(uint32_t n, uint32_t symmetry, const peps_options* opts,
+                                  peps_contraction_result* out, char* err, size_t err_cap);
+
+/* Count the completions of the k-row prefix cols[0..k) (queen of row i in
+ * column cols[i]).  An attacking prefix counts 0.  k = 0 is peps_contract_rows. */
+int peps_count_prefix(uint32_t n, const uint32_t* cols, uint32_t k, const peps_options* opts,
+                      peps_u128* count, char* err, size_t err_cap);
+
+/* Batched form: all prefix tasks of depth `split_depth` (reference
+ * build_wide_crt_tasks semantics: orbit weight on row 0) contracted as
+ * independent slices in grouped device launches; `out` receives the summed
+ * count and per-layer totals over the slices.  `tasks_out` (may be NULL)
+ * receives the number of tasks. */
+int peps_count_batched(uint32_t n, uint32_t split_depth, const peps_options* opts,
+                       peps_contraction_result* out, uint32_t* tasks_out, char* err,
+                       size_t err_cap);
+
+/* ------------------------------------------------------------ utilities -- */
+
+/* A000170 table Q(0..27); PEPS_E_UNKNOWN outside it. */
+int peps_known_count(uint32_t n, peps_u128* out);
+
+/* Peak resident set of this process in bytes (Linux getrusage). */
+uint64_t peps_peak_rss_bytes(void);
+
+/* CRT plan: the shortest prefix of the reference primes whose product exceeds
+ * N!.  Writes up to 4 primes and their count; returns PEPS_E_INVALID for N > 34. */
+int peps_crt_plan(uint32_t n, uint64_t* primes, uint32_t* count, peps_u128* modulus_product,
+                  char* err, size_t err_cap);
+
+/* Garner/Fermat reconstruction of residues[0..count) into a checked u128. */
+int peps_crt_reconstruct(const uint64_t* residues, const uint64_t* primes, uint32_t count,
+                         peps_u128* out, char* err, size_t err_cap);
+
+/* Prefix tasks of depth d: writes up to `cap` tasks as (columns, diag_dr,
+ * diag_dl, orbit_weight) quadruples into tasks[4*i..4*i+4) and the count into
+ * *count (which may exceed cap; then only cap are written). */
+int peps_prefix_tasks(uint32_t n, uint32_t depth, uint64_t* tasks, uint64_t cap, uint64_t* count,
+                      char* err, size_t err_cap);
+
+/* Device description: name (NUL-terminated, up to name_cap), CU count, HBM
+ * bytes, LDS bytes per workgroup.  PEPS_E_DEVICE without a GPU. */
+int peps_device_info(uint32_t device, char* name, size_t name_cap, uint32_t* cu_count,
+                     uint64_t* hbm_bytes, uint32_t* lds_bytes, char* err, size_t err_cap);
+
+/* Library version string. */
+const char* peps_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PEPS_MI355X_H */
