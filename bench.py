@@ -1,0 +1,204 @@
+"""Benchmark: exact N-Queens count by the explicit-C PEPS row contraction on
+one MI355X.
+
+A step is one full count at N (default 18): every row of the boundary
+contraction from the empty board to the final v1/v2 sum, with the reference's
+row-0 orbit slicing, on device-resident frontiers.  `value` is
+state-transitions per second, where transitions are the reference's
+`row_operator_matched` counter for that count (src/lib.rs:5370-5379) and the
+time is the wall time per count with the device warmed.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 18]
+                       [--form plain|batched] [--no-cpu-baseline]
+"""
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "quantum-harness-peps-naive_amd"))
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, vendor figure
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def host_cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
+
+
+def host_threads():
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
+def cpu_baseline(n, budget_s):
+    """The oracle's restatement of the reference's rayon path
+    (contract_rows_d4_orbit_parallel_sort_reduce, src/lib.rs:5469-5626) run on
+    a bounded sample: whole 2-row prefix tasks of the same board, timed until
+    the budget is spent.  Rate = transitions (row_operator_matched) / s."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import _oracle
+
+    threads = min(host_threads(), 64)
+    # the 2-row prefix tasks of the reference's task builder (row 0 left half)
+    tasks = [(c0, c1) for c0 in range((n + 1) // 2) for c1 in range(n)
+             if c1 != c0 and abs(c1 - c0) != 1]
+    order = [tasks[i] for i in range(0, len(tasks), max(1, len(tasks) // 8))]
+    done, matched, t_total = [], 0, 0.0
+    for cand in order:
+        t0 = time.perf_counter()
+        r = _oracle.contract_prefix(n, list(cand), threads=threads)
+        dt = time.perf_counter() - t0
+        t_total += dt
+        matched += r.row_operator_matched.value
+        done.append(list(cand))
+        log(f"cpu baseline task {cand}: {dt:.2f}s matched={r.row_operator_matched.value}")
+        if t_total >= budget_s:
+            break
+    return {
+        "value": matched / t_total if t_total else None,
+        "unit": "transitions/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": (f"{len(done)} of {len(tasks)} two-row prefix tasks of N={n} "
+                   f"(prefixes {done}) through the oracle's restatement of the reference's "
+                   f"rayon sort-reduce (checked u128 records, {threads} threads, {host_cpu_model()})"),
+        "seconds": t_total,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=18)
+    ap.add_argument("--form", choices=("plain", "batched"), default="plain")
+    ap.add_argument("--depth", type=int, default=2)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
+              f"(the contraction runs on a single MI355X; see the sharding note in DESIGN.md)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import peps_mi355x as pm
+
+    n = args.n
+    info = pm.device_info(0)
+    log(f"device: {info['name']} CUs={info['cu_count']} HBM={info['hbm_bytes'] / 2**30:.0f} GiB")
+
+    def step():
+        if args.form == "plain":
+            r = pm.contract_rows(n)
+        else:
+            r, _ = pm.count_batched(n, args.depth)
+        if r.count.value != pm.known_count(n):
+            raise SystemExit(f"count mismatch at N={n}: {r.count.value}")
+        return r
+
+    for i in range(args.warmup):
+        t0 = time.perf_counter()
+        r = step()
+        log(f"warmup {i}: {time.perf_counter() - t0:.3f}s count={r.count.value}")
+
+    # Transitions are the reference's counter for the plain (merged) path.
+    plain_matched = None
+    if args.form == "plain":
+        plain_matched = r.row_operator_matched.value if args.warmup else None
+
+    pm.profile_kernels(True)
+    times, last = [], None
+    t_all = time.perf_counter()
+    for i in range(args.steps):
+        t0 = time.perf_counter()
+        last = step()
+        times.append(time.perf_counter() - t0)
+        log(f"step {i}: {times[-1]:.3f}s kernel={last.kernel_s:.3f}s")
+    t_all = time.perf_counter() - t_all
+    stats = pm.kernel_stats()
+    pm.profile_kernels(False)
+    if plain_matched is None:
+        plain_matched = last.row_operator_matched.value if args.form == "plain" else None
+    if plain_matched is None:
+        plain_matched = pm.contract_rows(n).row_operator_matched.value
+
+    ms_per_step = 1000.0 * t_all / args.steps
+    value = plain_matched / (t_all / args.steps)
+
+    per_kernel = {}
+    for name, s in stats.items():
+        gbs = s["algorithmic_bytes"] / s["seconds"] / 1e9 if s["seconds"] else 0.0
+        per_kernel[name] = {"launches_per_step": s["launches"] / args.steps,
+                            "ms_per_step": 1000.0 * s["seconds"] / args.steps,
+                            "avg_launch_us": 1e6 * s["seconds"] / max(1, s["launches"]),
+                            "algorithmic_GBps": gbs}
+    dom = max(stats.items(), key=lambda kv: kv[1]["seconds"]) if stats else None
+    roofline = None
+    if dom:
+        name, s = dom
+        achieved = s["algorithmic_bytes"] / s["seconds"] / 1e9
+        roofline = {"bound": "hbm", "kernel": name, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                    "avg_launch_us": round(1e6 * s["seconds"] / s["launches"], 3),
+                    "algorithmic_bytes_per_launch": s["algorithmic_bytes"] / s["launches"]}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        try:
+            cpu = cpu_baseline(n, args.cpu_seconds)
+        except Exception as e:  # report, do not hide
+            cpu = {"value": None, "error": str(e)}
+
+    out = {
+        "metric": "state-transitions/s",
+        "value": value,
+        "unit": "transitions/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic (deterministic: the empty N x N board)",
+        "config": {"workload": f"exact N-Queens count, N={n}, "
+                               f"{'single merged frontier (plain drop-in)' if args.form == 'plain' else f'{args.depth}-row prefix slices (batched)'}"
+                               ", row-0 orbit slicing, CRT residues",
+                   "n": n, "form": args.form, "crt_lanes": last.crt_lanes,
+                   "transitions_per_count": plain_matched},
+        "time_to_count_s": min(times),
+        "time_to_count_median_s": sorted(times)[len(times) // 2],
+        "kernel_s": last.kernel_s,
+        "count": last.count.value,
+        "peak_states": last.peak_states,
+        "device_peak_bytes": last.device_peak_bytes,
+        "per_kernel": per_kernel,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    if cpu and cpu.get("value"):
+        out["cpu_time_to_count_est_s"] = plain_matched / cpu["value"]
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -174,6 +174,23 @@ int peps_device_info(uint32_t device, char* name, size_t name_cap, uint32_t* cu_
 /* Library version string. */
 const char* peps_version(void);
 
+/* ------------------------------------------------------------- profiling -- */
+
+/* Per-kernel timing gathered with HIP events on the engine's stream. */
+typedef struct {
+  char name[32];
+  uint64_t launches;
+  double seconds;           /* summed event-to-event device time */
+  double algorithmic_bytes; /* compulsory HBM bytes of those launches */
+} peps_kernel_stat;
+
+/* Enable (nonzero) or disable kernel timing on `device`; either resets the stats. */
+int peps_profile_kernels(uint32_t device, uint32_t enable, char* err, size_t err_cap);
+
+/* Copy up to `cap` kernel stats; *count receives how many kernels have stats. */
+int peps_kernel_stats(uint32_t device, peps_kernel_stat* out, uint32_t cap, uint32_t* count,
+                      char* err, size_t err_cap);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,446 @@
+// capi.cpp -- the C ABI of libpeps_mi355x.so (include/peps_mi355x.h).
+//
+// Mirrors the reference's library surface (src/lib.rs): contract_rows and its
+// symmetry variants, known_count, peak_rss_bytes, the CRT plan and
+// reconstruction, and the prefix-task builder; plus the prefix count and the
+// batched (sliced) form.  Every contraction runs on the GPU engine; there is
+// no CPU fallback: without a gfx950 device the calls fail with PEPS_E_DEVICE.
+#include <sys/resource.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/peps_mi355x.h"
+#include "engine.h"
+#include "host_math.h"
+
+using peps::u128;
+
+namespace {
+
+int fail(int code, const std::string& msg, char* err, size_t cap) {
+  if (err && cap) {
+    size_t n = std::min(cap - 1, msg.size());
+    std::memcpy(err, msg.data(), n);
+    err[n] = 0;
+  }
+  return code;
+}
+
+peps_u128 abi(u128 v) { return peps_u128{(uint64_t)v, (uint64_t)(v >> 64)}; }
+
+uint64_t rss_bytes() {
+  struct rusage ru;
+  if (getrusage(RUSAGE_SELF, &ru) != 0) return 0;
+  return (uint64_t)ru.ru_maxrss * 1024ull;
+}
+
+struct Resolved {
+  std::vector<uint64_t> primes;
+  uint32_t radix_bits = 8;
+  uint32_t tile_states = 2048;
+  uint32_t device = 0;
+  uint64_t budget = 0;
+};
+
+// Lane count: the reference plan (src/lib.rs:3859-3886) unless the caller asks
+// for more lanes; fewer lanes than the plan would not certify exactness.
+int resolve(unsigned n, const peps_options* o, Resolved* r, std::string* err) {
+  u128 prod, fact;
+  if (!peps::crt_plan(n, &r->primes, &prod, &fact, err)) return PEPS_E_INVALID;
+  if (o) {
+    if (o->crt_lanes) {
+      if (o->crt_lanes > PEPS_MAX_LANES) {
+        *err = "crt_lanes must be at most 4";
+        return PEPS_E_INVALID;
+      }
+      if (o->crt_lanes < r->primes.size()) {
+        *err = "crt_lanes below the certified plan (product of primes must exceed N!)";
+        return PEPS_E_INVALID;
+      }
+      r->primes.assign(peps::crt_primes(), peps::crt_primes() + o->crt_lanes);
+    }
+    if (o->radix_bits) {
+      if (o->radix_bits < 1 || o->radix_bits > 10) {
+        *err = "radix_bits must be in 1..10";
+        return PEPS_E_INVALID;
+      }
+      r->radix_bits = o->radix_bits;
+    }
+    if (o->tile_states) {
+      if (o->tile_states < 256 || o->tile_states > 2560) {
+        *err = "tile_states must be in 256..2560";
+        return PEPS_E_INVALID;
+      }
+      r->tile_states = o->tile_states;
+    }
+    r->device = o->device;
+    r->budget = o->hbm_budget_bytes;
+  }
+  return PEPS_OK;
+}
+
+void fill_result(unsigned n, const peps::RunOut& ro, const std::vector<uint64_t>& primes,
+                 double elapsed, peps_contraction_result* out) {
+  out->n = n;
+  out->count = abi(ro.count);
+  out->elapsed_s = elapsed;
+  out->peak_states = ro.peak_states;
+  out->tensor_entries_examined = abi(17);
+  out->tensor_entries_matched = abi(17);
+  out->row_operator_candidates = abi(ro.total_candidates);
+  out->row_operator_matched = abi(ro.total_matched);
+  out->layer_count = (uint32_t)std::min<size_t>(ro.layers.size(), PEPS_MAX_N);
+  for (uint32_t i = 0; i < out->layer_count; ++i) {
+    const peps::LayerOut& l = ro.layers[i];
+    peps_layer_metric& m = out->layers[i];
+    m.row = l.row;
+    m.input_states = l.input_states;
+    m.tensor_entries_examined = abi(0);
+    m.tensor_entries_matched = abi(0);
+    m.row_operator_candidates = abi(l.candidates);
+    m.row_operator_matched = abi(l.matched);
+    m.completed_row_terms = abi(l.completed);
+    m.output_states = l.output_states;
+    m.output_weight = abi(l.output_weight);
+    m.elapsed_s = l.elapsed_s;
+    m.peak_rss_bytes = rss_bytes();
+    m.device_bytes = l.device_bytes;
+    m.partition_levels = l.levels;
+    m.final_buckets = l.buckets;
+  }
+  out->crt_lanes = (uint32_t)primes.size();
+  for (size_t l = 0; l < primes.size() && l < PEPS_MAX_LANES; ++l) {
+    out->primes[l] = primes[l];
+    out->residues[l] = l < ro.residues.size() ? ro.residues[l] : 0;
+  }
+  out->device_peak_bytes = ro.device_peak_bytes;
+  out->kernel_s = ro.kernel_s;
+  out->peak_rss_bytes = rss_bytes();
+}
+
+int check_board(unsigned n, std::string* err) {
+  if (n > 42) {
+    *err = "the packed u128 virtual-boundary backend supports N <= 42";
+    return PEPS_E_INVALID;
+  }
+  if (n > PEPS_MAX_DEVICE_N) {
+    *err = "the MI355X boundary key (u64, 3N bits) supports N <= 21";
+    return PEPS_E_INVALID;
+  }
+  return PEPS_OK;
+}
+
+int run_engine(const Resolved& r, peps::RunSpec& spec, peps::RunOut* ro, std::string* err) {
+  peps::Engine* eng = peps::Engine::get((int)r.device, err);
+  if (!eng) return PEPS_E_DEVICE;
+  spec.primes = r.primes;
+  spec.radix_bits = r.radix_bits;
+  spec.tile_states = r.tile_states;
+  spec.budget_bytes = r.budget;
+  std::lock_guard<std::mutex> lk(eng->mutex());
+  return eng->run(spec, ro, err);
+}
+
+// Boundary after a k-row prefix; false if a queen attacks an earlier one.
+bool prefix_state(unsigned n, const uint32_t* cols, unsigned k, uint64_t* key) {
+  const uint64_t bm = ((uint64_t)1 << n) - 1;
+  uint64_t c = 0, dr = 0, dl = 0;
+  for (unsigned i = 0; i < k; ++i) {
+    const uint64_t b = (uint64_t)1 << cols[i];
+    if ((c | dr | dl) & b) return false;
+    c |= b;
+    dr = ((dr | b) << 1) & bm;
+    dl = (dl | b) >> 1;
+  }
+  *key = c | (dr << n) | (dl << (2 * n));
+  return true;
+}
+
+struct Task {
+  uint64_t key;
+  uint64_t weight;
+};
+
+// build_wide_crt_tasks (src/lib.rs:3888-3956) for an explicit depth: expand
+// the compiled relation row by row without merging, applying the orbit weight
+// on row 0.
+std::vector<Task> make_tasks(unsigned n, unsigned depth) {
+  const uint64_t bm = n ? ((uint64_t)1 << n) - 1 : 0;
+  std::vector<Task> tasks{Task{0, 1}};
+  for (unsigned d = 0; d < depth && d < n; ++d) {
+    std::vector<Task> next;
+    for (const Task& t : tasks) {
+      const uint64_t c = t.key & bm, dr = (t.key >> n) & bm, dl = (t.key >> (2 * n)) & bm;
+      uint64_t free = ~(c | dr | dl) & bm;
+      while (free) {
+        const unsigned col = (unsigned)__builtin_ctzll(free);
+        free &= free - 1;
+        const uint64_t b = (uint64_t)1 << col;
+        uint64_t w = 1;
+        if (d == 0) {
+          const unsigned mirror = n - 1 - col;
+          if (col > mirror) continue;
+          w = col == mirror ? 1 : 2;
+        }
+        const uint64_t nc = c | b, ndr = ((dr | b) << 1) & bm, ndl = (dl | b) >> 1;
+        next.push_back(Task{nc | (ndr << n) | (ndl << (2 * n)), t.weight * w});
+      }
+    }
+    tasks.swap(next);
+  }
+  return tasks;
+}
+
+}  // namespace
+
+extern "C" {
+
+int peps_contract_rows_with_modes(uint32_t n, uint32_t symmetry, const peps_options* opts,
+                                  peps_contraction_result* out, char* err, size_t err_cap) {
+  if (!out) return fail(PEPS_E_INVALID, "null result pointer", err, err_cap);
+  std::memset(out, 0, sizeof(*out));
+  out->n = n;
+  std::string e;
+  if (symmetry > PEPS_SYMMETRY_TOP_ROW_ORBITS) return fail(PEPS_E_INVALID, "unknown symmetry mode", err, err_cap);
+  if (n > 42) return fail(PEPS_E_INVALID, "the packed u128 virtual-boundary backend supports N <= 42", err, err_cap);
+  if (n == 0) {  // src/lib.rs:5295-5307
+    out->count = abi(1);
+    out->peak_states = 1;
+    out->peak_rss_bytes = rss_bytes();
+    return PEPS_OK;
+  }
+  int rc = check_board(n, &e);
+  if (rc) return fail(rc, e, err, err_cap);
+  Resolved r;
+  rc = resolve(n, opts, &r, &e);
+  if (rc) return fail(rc, e, err, err_cap);
+  peps::RunSpec spec;
+  spec.n = n;
+  spec.first_row = 0;
+  spec.orbit_first_row = symmetry == PEPS_SYMMETRY_TOP_ROW_ORBITS;
+  spec.seeds.push_back(peps::Seed{0, 1});
+  peps::RunOut ro;
+  const auto t0 = std::chrono::steady_clock::now();
+  rc = run_engine(r, spec, &ro, &e);
+  if (rc) return fail(rc, e, err, err_cap);
+  const double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  fill_result(n, ro, r.primes, el, out);
+  return PEPS_OK;
+}
+
+int peps_contract_rows(uint32_t n, const peps_options* opts, peps_contraction_result* out,
+                       char* err, size_t err_cap) {
+  return peps_contract_rows_with_modes(n, PEPS_SYMMETRY_TOP_ROW_ORBITS, opts, out, err, err_cap);
+}
+
+int peps_count_prefix(uint32_t n, const uint32_t* cols, uint32_t k, const peps_options* opts,
+                      peps_u128* count, char* err, size_t err_cap) {
+  if (!count) return fail(PEPS_E_INVALID, "null count pointer", err, err_cap);
+  *count = abi(0);
+  std::string e;
+  if (k > n) return fail(PEPS_E_INVALID, "prefix longer than the board", err, err_cap);
+  if (k && !cols) return fail(PEPS_E_INVALID, "null prefix", err, err_cap);
+  for (uint32_t i = 0; i < k; ++i)
+    if (cols[i] >= n) return fail(PEPS_E_INVALID, "prefix column outside the board", err, err_cap);
+  if (k == 0) {
+    peps_contraction_result res;
+    int rc = peps_contract_rows(n, opts, &res, err, err_cap);
+    if (rc == PEPS_OK) *count = res.count;
+    return rc;
+  }
+  int rc = check_board(n, &e);
+  if (rc) return fail(rc, e, err, err_cap);
+  uint64_t key;
+  if (!prefix_state(n, cols, k, &key)) return PEPS_OK;  // attacking prefix: C gives 0
+  if (k == n) {
+    *count = abi(1);
+    return PEPS_OK;
+  }
+  Resolved r;
+  rc = resolve(n, opts, &r, &e);
+  if (rc) return fail(rc, e, err, err_cap);
+  peps::RunSpec spec;
+  spec.n = n;
+  spec.first_row = k;
+  spec.seeds.push_back(peps::Seed{key, 1});
+  peps::RunOut ro;
+  rc = run_engine(r, spec, &ro, &e);
+  if (rc) return fail(rc, e, err, err_cap);
+  *count = abi(ro.count);
+  return PEPS_OK;
+}
+
+int peps_count_batched(uint32_t n, uint32_t split_depth, const peps_options* opts,
+                       peps_contraction_result* out, uint32_t* tasks_out, char* err,
+                       size_t err_cap) {
+  if (!out) return fail(PEPS_E_INVALID, "null result pointer", err, err_cap);
+  std::memset(out, 0, sizeof(*out));
+  out->n = n;
+  std::string e;
+  if (n == 0 || split_depth == 0 || split_depth >= n) {
+    if (n > 42) return fail(PEPS_E_INVALID, "the packed u128 virtual-boundary backend supports N <= 42", err, err_cap);
+    if (tasks_out) *tasks_out = 1;
+    return peps_contract_rows(n, opts, out, err, err_cap);
+  }
+  int rc = check_board(n, &e);
+  if (rc) return fail(rc, e, err, err_cap);
+  Resolved r;
+  rc = resolve(n, opts, &r, &e);
+  if (rc) return fail(rc, e, err, err_cap);
+  std::vector<Task> tasks = make_tasks(n, split_depth);
+  if (tasks_out) *tasks_out = (uint32_t)tasks.size();
+  const unsigned free_bits = 63 - 3 * n;
+  const size_t group = std::min<size_t>(tasks.size(), (size_t)1 << std::min(free_bits, 20u));
+  const auto t0 = std::chrono::steady_clock::now();
+  peps::RunOut total;
+  total.residues.assign(r.primes.size(), 0);
+  total.peak_states = 0;
+  std::vector<peps::LayerOut> layers;
+  for (size_t g0 = 0; g0 < tasks.size(); g0 += group) {
+    const size_t g1 = std::min(tasks.size(), g0 + group);
+    peps::RunSpec spec;
+    spec.n = n;
+    spec.first_row = split_depth;
+    unsigned bits = 0;
+    while (((size_t)1 << bits) < g1 - g0) ++bits;
+    spec.slice_bits = bits;
+    for (size_t i = g0; i < g1; ++i)
+      spec.seeds.push_back(peps::Seed{tasks[i].key | ((uint64_t)(i - g0) << (3 * n)), tasks[i].weight});
+    peps::RunOut ro;
+    rc = run_engine(r, spec, &ro, &e);
+    if (rc) return fail(rc, e, err, err_cap);
+    if (!peps::checked_add(total.count, ro.count, &total.count))
+      return fail(PEPS_E_OVERFLOW, "batched count overflow", err, err_cap);
+    total.peak_states = std::max(total.peak_states, ro.peak_states);
+    total.total_candidates += ro.total_candidates;
+    total.total_matched += ro.total_matched;
+    total.kernel_s += ro.kernel_s;
+    total.device_peak_bytes = std::max(total.device_peak_bytes, ro.device_peak_bytes);
+    if (layers.empty()) {
+      layers = ro.layers;
+    } else {
+      for (size_t i = 0; i < layers.size() && i < ro.layers.size(); ++i) {
+        layers[i].input_states += ro.layers[i].input_states;
+        layers[i].candidates += ro.layers[i].candidates;
+        layers[i].matched += ro.layers[i].matched;
+        layers[i].completed += ro.layers[i].completed;
+        layers[i].output_states += ro.layers[i].output_states;
+        layers[i].output_weight += ro.layers[i].output_weight;
+        layers[i].elapsed_s += ro.layers[i].elapsed_s;
+      }
+    }
+  }
+  total.layers = layers;
+  u128 fact;
+  peps::factorial(n, &fact);
+  if (total.count > fact) return fail(PEPS_E_OVERFLOW, "batched count exceeds N!", err, err_cap);
+  for (size_t l = 0; l < r.primes.size(); ++l) total.residues[l] = (uint64_t)(total.count % r.primes[l]);
+  const double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  fill_result(n, total, r.primes, el, out);
+  return PEPS_OK;
+}
+
+int peps_known_count(uint32_t n, peps_u128* out) {
+  u128 v;
+  if (!peps::known_count(n, &v)) return PEPS_E_UNKNOWN;
+  if (out) *out = abi(v);
+  return PEPS_OK;
+}
+
+uint64_t peps_peak_rss_bytes(void) { return rss_bytes(); }
+
+int peps_crt_plan(uint32_t n, uint64_t* primes, uint32_t* count, peps_u128* modulus_product,
+                  char* err, size_t err_cap) {
+  std::vector<uint64_t> p;
+  u128 prod, fact;
+  std::string e;
+  if (!peps::crt_plan(n, &p, &prod, &fact, &e)) return fail(PEPS_E_INVALID, e, err, err_cap);
+  for (size_t i = 0; i < p.size(); ++i)
+    if (primes) primes[i] = p[i];
+  if (count) *count = (uint32_t)p.size();
+  if (modulus_product) *modulus_product = abi(prod);
+  return PEPS_OK;
+}
+
+int peps_crt_reconstruct(const uint64_t* residues, const uint64_t* primes, uint32_t count,
+                         peps_u128* out, char* err, size_t err_cap) {
+  if (!residues || !primes || !out || count > PEPS_MAX_LANES)
+    return fail(PEPS_E_INVALID, "CRT residue/modulus length mismatch", err, err_cap);
+  u128 v;
+  std::string e;
+  if (!peps::crt_reconstruct(residues, primes, count, &v, &e)) return fail(PEPS_E_OVERFLOW, e, err, err_cap);
+  *out = abi(v);
+  return PEPS_OK;
+}
+
+int peps_prefix_tasks(uint32_t n, uint32_t depth, uint64_t* tasks, uint64_t cap, uint64_t* count,
+                      char* err, size_t err_cap) {
+  if (n > PEPS_MAX_DEVICE_N)
+    return fail(PEPS_E_INVALID, "prefix tasks support N <= 21", err, err_cap);
+  std::vector<Task> t = make_tasks(n, depth);
+  if (count) *count = t.size();
+  const uint64_t bm = n ? ((uint64_t)1 << n) - 1 : 0;
+  for (uint64_t i = 0; i < t.size() && i < cap && tasks; ++i) {
+    tasks[4 * i + 0] = t[i].key & bm;
+    tasks[4 * i + 1] = (t[i].key >> n) & bm;
+    tasks[4 * i + 2] = (t[i].key >> (2 * n)) & bm;
+    tasks[4 * i + 3] = t[i].weight;
+  }
+  return PEPS_OK;
+}
+
+int peps_device_info(uint32_t device, char* name, size_t name_cap, uint32_t* cu_count,
+                     uint64_t* hbm_bytes, uint32_t* lds_bytes, char* err, size_t err_cap) {
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count == 0)
+    return fail(PEPS_E_DEVICE, "no HIP device available", err, err_cap);
+  if ((int)device >= count) return fail(PEPS_E_INVALID, "HIP device ordinal out of range", err, err_cap);
+  hipDeviceProp_t p;
+  if (hipGetDeviceProperties(&p, (int)device) != hipSuccess)
+    return fail(PEPS_E_DEVICE, "hipGetDeviceProperties failed", err, err_cap);
+  if (name && name_cap) {
+    std::string s = std::string(p.name) + " (" + p.gcnArchName + ")";
+    size_t k = std::min(name_cap - 1, s.size());
+    std::memcpy(name, s.data(), k);
+    name[k] = 0;
+  }
+  if (cu_count) *cu_count = (uint32_t)p.multiProcessorCount;
+  if (hbm_bytes) *hbm_bytes = (uint64_t)p.totalGlobalMem;
+  if (lds_bytes) *lds_bytes = (uint32_t)p.sharedMemPerBlock;
+  return PEPS_OK;
+}
+
+const char* peps_version(void) { return "peps_mi355x 0.1.0 (gfx950)"; }
+
+int peps_profile_kernels(uint32_t device, uint32_t enable, char* err, size_t err_cap) {
+  std::string e;
+  peps::Engine* eng = peps::Engine::get((int)device, &e);
+  if (!eng) return fail(PEPS_E_DEVICE, e, err, err_cap);
+  std::lock_guard<std::mutex> lk(eng->mutex());
+  eng->set_profiling(enable != 0);
+  return PEPS_OK;
+}
+
+int peps_kernel_stats(uint32_t device, peps_kernel_stat* out, uint32_t cap, uint32_t* count,
+                      char* err, size_t err_cap) {
+  std::string e;
+  peps::Engine* eng = peps::Engine::get((int)device, &e);
+  if (!eng) return fail(PEPS_E_DEVICE, e, err, err_cap);
+  std::lock_guard<std::mutex> lk(eng->mutex());
+  auto v = eng->kernel_stats();
+  if (count) *count = (uint32_t)v.size();
+  for (uint32_t i = 0; i < v.size() && i < cap && out; ++i) {
+    std::memset(out[i].name, 0, sizeof(out[i].name));
+    std::strncpy(out[i].name, v[i].first.c_str(), sizeof(out[i].name) - 1);
+    out[i].launches = v[i].second.launches;
+    out[i].seconds = v[i].second.seconds;
+    out[i].algorithmic_bytes = v[i].second.bytes;
+  }
+  return PEPS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,125 @@
+// engine.h -- device-resident row contraction driver (one per device).
+//
+// The engine owns a HIP stream and an HBM arena of three regions whose roles
+// rotate every row: the frontier (SoA: u64 keys, then one u32 residue array
+// per CRT prime) and two partition buffers (AoS records of the radix
+// partition).  A row is: count -> plan the hash-radix levels -> expand
+// (histogram, scan, scatter) -> partition levels -> LDS merge, which writes
+// the next frontier into the free partition buffer.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "host_math.h"
+#include "kernels.h"
+
+namespace peps {
+
+struct Seed {
+  uint64_t key;  // packed boundary, slice id above bit 3n
+  u128 weight;
+};
+
+struct RunSpec {
+  unsigned n = 0;
+  unsigned first_row = 0;          // contract rows [first_row, n)
+  bool orbit_first_row = false;    // row-0 vertical-reflection orbit weights
+  std::vector<Seed> seeds;         // frontier entering first_row (distinct keys)
+  unsigned slice_bits = 0;         // slice ids occupy bits [3n, 3n + slice_bits)
+  std::vector<uint64_t> primes;    // CRT lanes (1..4)
+  uint32_t tile_states = 2048;     // target records per LDS merge bucket
+  uint32_t radix_bits = 8;         // max hash-radix digit width per level
+  uint64_t budget_bytes = 0;       // arena cap (0: 85% of free HBM at first use)
+};
+
+struct LayerOut {
+  unsigned row;
+  uint64_t input_states;
+  u128 candidates;  // dense row operator candidates, n per parent
+  u128 matched;     // legal (parent, column) pairs before the orbit filter
+  u128 completed;   // successor terms after the orbit filter
+  uint64_t output_states;
+  u128 output_weight;
+  double elapsed_s;
+  uint64_t device_bytes;
+  uint32_t levels;
+  uint32_t buckets;
+};
+
+struct RunOut {
+  u128 count = 0;
+  std::vector<uint64_t> residues;
+  std::vector<u128> slice_counts;  // per slice id when slice_bits > 0
+  std::vector<LayerOut> layers;
+  uint64_t peak_states = 1;
+  u128 total_candidates = 0, total_matched = 0;
+  double elapsed_s = 0, kernel_s = 0;
+  uint64_t device_peak_bytes = 0;
+};
+
+struct KernelStat {
+  uint64_t launches = 0;
+  double seconds = 0;
+  double bytes = 0;
+};
+
+class Engine {
+ public:
+  static Engine* get(int device, std::string* err);  // one engine per device
+  int run(const RunSpec& spec, RunOut* out, std::string* err);
+  std::mutex& mutex() { return mu_; }
+  int device() const { return device_; }
+  // Kernel timing with HIP events on the engine stream (off by default).
+  void set_profiling(bool on);
+  std::vector<std::pair<std::string, KernelStat>> kernel_stats() const;
+
+ private:
+  explicit Engine(int device) : device_(device) {}
+  int init(std::string* err);
+  int ensure_region(int idx, size_t bytes, std::string* err);
+  template <typename T>
+  int ensure_buffer(T** ptr, size_t* cap, size_t count, std::string* err);
+  uint64_t arena_bytes() const;
+
+  int device_;
+  std::mutex mu_;
+  hipStream_t stream_ = nullptr;
+  hipEvent_t ev_start_ = nullptr, ev_stop_ = nullptr;
+  void* region_[3] = {nullptr, nullptr, nullptr};
+  size_t region_bytes_[3] = {0, 0, 0};
+  uint64_t* counters_ = nullptr;   // device, kNumCounters
+  uint64_t* h_counters_ = nullptr; // pinned host mirror
+  uint32_t* hist_ = nullptr;
+  size_t hist_cap_ = 0;
+  uint64_t* offsets_ = nullptr;
+  size_t offsets_cap_ = 0;
+  uint64_t* segs_[2] = {nullptr, nullptr};
+  size_t segs_cap_[2] = {0, 0};
+  uint64_t* slice_sums_ = nullptr;
+  size_t slice_cap_ = 0;
+  uint64_t budget_ = 0;
+  Legs occupied_{};
+
+  // profiling
+  struct Pending {
+    int kernel;
+    double bytes;
+    hipEvent_t a, b;
+  };
+  bool profiling_ = false;
+  std::vector<hipEvent_t> event_pool_;
+  std::vector<Pending> pending_;
+  std::vector<std::string> kernel_names_;
+  std::vector<KernelStat> kstats_;
+  hipError_t mark_begin(hipEvent_t* ev);
+  hipError_t mark_end(int kernel, double bytes, hipEvent_t begin);
+  int kernel_id(const char* name);
+  int resolve_pending(std::string* err);
+};
+
+}  // namespace peps

@@ -109,3 +109,12 @@ class Options(ctypes.Structure):
         ("collect_layers", ctypes.c_uint32),
         ("reserved", ctypes.c_uint32 * 7),
     ]
+
+
+class KernelStat(ctypes.Structure):
+    _fields_ = [
+        ("name", ctypes.c_char * 32),
+        ("launches", ctypes.c_uint64),
+        ("seconds", ctypes.c_double),
+        ("algorithmic_bytes", ctypes.c_double),
+    ]

@@ -1,0 +1,216 @@
+"""ctypes bindings of libpeps_mi355x.so with the reference's function names.
+
+Reference functions mirrored (crate nqueens-peps-naive, src/lib.rs):
+  contract_rows                       src/lib.rs:757-760
+  contract_rows_sort_reduce           src/lib.rs:770-772  (no orbit slicing)
+  contract_rows_d4_orbit_sort_reduce  src/lib.rs:784-786
+  known_count                         src/lib.rs:5752-5784
+  peak_rss_bytes                      src/lib.rs:5786-5850
+  wide_crt_plan / reconstruct_crt     src/lib.rs:3859-3886, 4134-4163
+Errors surface as PepsError carrying the library's message (the reference
+returns Err(String) with the same texts).
+"""
+import ctypes
+import dataclasses
+import os
+
+from .abi import (
+    PEPS_SYMMETRY_NONE,
+    PEPS_SYMMETRY_TOP_ROW_ORBITS,
+    U128,
+    ContractionResult,
+    KernelStat,
+    Options,
+)
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_NAME = "libpeps_mi355x.so"
+_lib = None
+
+
+class PepsError(RuntimeError):
+    """A contraction or argument error reported by the library."""
+
+    def __init__(self, code, message):
+        super().__init__(message)
+        self.code = code
+
+
+@dataclasses.dataclass
+class PepsOptions:
+    crt_lanes: int = 0          # 0: reference CRT plan
+    radix_bits: int = 0         # 0: default hash-radix digit width
+    tile_states: int = 0        # 0: default LDS merge bucket size
+    device: int = 0
+    hbm_budget_bytes: int = 0
+
+    def to_c(self) -> Options:
+        o = Options()
+        o.crt_lanes = self.crt_lanes
+        o.radix_bits = self.radix_bits
+        o.tile_states = self.tile_states
+        o.device = self.device
+        o.hbm_budget_bytes = self.hbm_budget_bytes
+        o.collect_layers = 1
+        return o
+
+
+def lib_path() -> str:
+    return os.environ.get("PEPS_MI355X_LIB", os.path.join(_HERE, _LIB_NAME))
+
+
+def load():
+    """Load the HIP library; raises PepsError if it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    path = lib_path()
+    if not os.path.exists(path):
+        raise PepsError(3, f"libpeps_mi355x.so is not built at {path}; run __graft_entry__.build()")
+    L = ctypes.CDLL(path)
+    c = ctypes
+    P = c.POINTER
+    L.peps_contract_rows.argtypes = [c.c_uint32, P(Options), P(ContractionResult), c.c_char_p, c.c_size_t]
+    L.peps_contract_rows_with_modes.argtypes = [c.c_uint32, c.c_uint32, P(Options), P(ContractionResult),
+                                                c.c_char_p, c.c_size_t]
+    L.peps_count_prefix.argtypes = [c.c_uint32, P(c.c_uint32), c.c_uint32, P(Options), P(U128),
+                                    c.c_char_p, c.c_size_t]
+    L.peps_count_batched.argtypes = [c.c_uint32, c.c_uint32, P(Options), P(ContractionResult),
+                                     P(c.c_uint32), c.c_char_p, c.c_size_t]
+    L.peps_known_count.argtypes = [c.c_uint32, P(U128)]
+    L.peps_peak_rss_bytes.restype = c.c_uint64
+    L.peps_crt_plan.argtypes = [c.c_uint32, P(c.c_uint64), P(c.c_uint32), P(U128), c.c_char_p, c.c_size_t]
+    L.peps_crt_reconstruct.argtypes = [P(c.c_uint64), P(c.c_uint64), c.c_uint32, P(U128), c.c_char_p,
+                                       c.c_size_t]
+    L.peps_prefix_tasks.argtypes = [c.c_uint32, c.c_uint32, P(c.c_uint64), c.c_uint64, P(c.c_uint64),
+                                    c.c_char_p, c.c_size_t]
+    L.peps_device_info.argtypes = [c.c_uint32, c.c_char_p, c.c_size_t, P(c.c_uint32), P(c.c_uint64),
+                                   P(c.c_uint32), c.c_char_p, c.c_size_t]
+    L.peps_version.restype = c.c_char_p
+    L.peps_profile_kernels.argtypes = [c.c_uint32, c.c_uint32, c.c_char_p, c.c_size_t]
+    L.peps_kernel_stats.argtypes = [c.c_uint32, P(KernelStat), c.c_uint32, P(c.c_uint32), c.c_char_p,
+                                    c.c_size_t]
+    _lib = L
+    return L
+
+
+def _err():
+    return ctypes.create_string_buffer(1024)
+
+
+def _check(rc, e):
+    if rc != 0:
+        raise PepsError(rc, e.value.decode(errors="replace"))
+
+
+def _opts(options):
+    o = (options or PepsOptions()).to_c()
+    return ctypes.byref(o), o
+
+
+def contract_rows_with_modes(n: int, symmetry: int, options: PepsOptions = None) -> ContractionResult:
+    out = ContractionResult()
+    e = _err()
+    ref, _keep = _opts(options)
+    _check(load().peps_contract_rows_with_modes(n, symmetry, ref, ctypes.byref(out), e, len(e)), e)
+    return out
+
+
+def contract_rows(n: int, options: PepsOptions = None) -> ContractionResult:
+    """Reference contract_rows(n): orbit-sliced exact row contraction."""
+    return contract_rows_with_modes(n, PEPS_SYMMETRY_TOP_ROW_ORBITS, options)
+
+
+def contract_rows_d4_orbit_sort_reduce(n: int, options: PepsOptions = None) -> ContractionResult:
+    return contract_rows_with_modes(n, PEPS_SYMMETRY_TOP_ROW_ORBITS, options)
+
+
+def contract_rows_sort_reduce(n: int, options: PepsOptions = None) -> ContractionResult:
+    """Reference contract_rows_sort_reduce(n): no orbit slicing."""
+    return contract_rows_with_modes(n, PEPS_SYMMETRY_NONE, options)
+
+
+def count_prefix(n: int, cols, options: PepsOptions = None) -> int:
+    cols = list(cols)
+    arr = (ctypes.c_uint32 * max(1, len(cols)))(*cols)
+    out = U128()
+    e = _err()
+    ref, _keep = _opts(options)
+    _check(load().peps_count_prefix(n, arr, len(cols), ref, ctypes.byref(out), e, len(e)), e)
+    return out.value
+
+
+def count_batched(n: int, split_depth: int = 2, options: PepsOptions = None):
+    out = ContractionResult()
+    tasks = ctypes.c_uint32()
+    e = _err()
+    ref, _keep = _opts(options)
+    _check(load().peps_count_batched(n, split_depth, ref, ctypes.byref(out), ctypes.byref(tasks), e, len(e)), e)
+    return out, tasks.value
+
+
+def known_count(n: int):
+    v = U128()
+    return v.value if load().peps_known_count(n, ctypes.byref(v)) == 0 else None
+
+
+def peak_rss_bytes() -> int:
+    return int(load().peps_peak_rss_bytes())
+
+
+def crt_plan(n: int):
+    primes = (ctypes.c_uint64 * 4)()
+    cnt = ctypes.c_uint32()
+    prod = U128()
+    e = _err()
+    _check(load().peps_crt_plan(n, primes, ctypes.byref(cnt), ctypes.byref(prod), e, len(e)), e)
+    return list(primes)[: cnt.value], prod.value
+
+
+def crt_reconstruct(residues, primes) -> int:
+    r = (ctypes.c_uint64 * max(1, len(residues)))(*residues)
+    p = (ctypes.c_uint64 * max(1, len(primes)))(*primes)
+    out = U128()
+    e = _err()
+    _check(load().peps_crt_reconstruct(r, p, len(residues), ctypes.byref(out), e, len(e)), e)
+    return out.value
+
+
+def prefix_tasks(n: int, depth: int):
+    cnt = ctypes.c_uint64()
+    e = _err()
+    _check(load().peps_prefix_tasks(n, depth, None, 0, ctypes.byref(cnt), e, len(e)), e)
+    buf = (ctypes.c_uint64 * (4 * max(1, cnt.value)))()
+    _check(load().peps_prefix_tasks(n, depth, buf, cnt.value, ctypes.byref(cnt), e, len(e)), e)
+    return [tuple(buf[4 * i: 4 * i + 4]) for i in range(cnt.value)]
+
+
+def device_info(device: int = 0) -> dict:
+    name = ctypes.create_string_buffer(256)
+    cu, hbm, lds = ctypes.c_uint32(), ctypes.c_uint64(), ctypes.c_uint32()
+    e = _err()
+    _check(load().peps_device_info(device, name, 256, ctypes.byref(cu), ctypes.byref(hbm), ctypes.byref(lds),
+                                   e, len(e)), e)
+    return {"name": name.value.decode(), "cu_count": cu.value, "hbm_bytes": hbm.value, "lds_bytes": lds.value}
+
+
+def version() -> str:
+    return load().peps_version().decode()
+
+
+def profile_kernels(enable: bool, device: int = 0) -> None:
+    """Turn HIP-event kernel timing on or off (either resets the stats)."""
+    e = _err()
+    _check(load().peps_profile_kernels(device, int(enable), e, len(e)), e)
+
+
+def kernel_stats(device: int = 0) -> dict:
+    """{kernel: {launches, seconds, algorithmic_bytes}} since the last reset."""
+    cap = 32
+    buf = (KernelStat * cap)()
+    cnt = ctypes.c_uint32()
+    e = _err()
+    _check(load().peps_kernel_stats(device, buf, cap, ctypes.byref(cnt), e, len(e)), e)
+    return {buf[i].name.decode(): {"launches": buf[i].launches, "seconds": buf[i].seconds,
+                                   "algorithmic_bytes": buf[i].algorithmic_bytes}
+            for i in range(min(cap, cnt.value))}
