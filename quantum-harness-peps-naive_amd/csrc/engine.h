@@ -1,11 +1,11 @@
 // engine.h -- device-resident row contraction driver (one per device).
 //
-// The engine owns a HIP stream and an HBM arena of three regions whose roles
-// rotate every row: the frontier (SoA: u64 keys, then one u32 residue array
-// per CRT prime) and two partition buffers (AoS records of the radix
-// partition).  A row is: count -> plan the hash-radix levels -> expand
-// (histogram, scan, scatter) -> partition levels -> LDS merge, which writes
-// the next frontier into the free partition buffer.
+// The engine owns a HIP stream and an HBM arena of two candidate buffers
+// whose roles alternate every row.  The seed frontier (SoA: u64 keys, then one
+// u32 residue array per CRT prime) is expanded into the first row's
+// hash-radix buckets; from then on each row's buckets are merged and expanded
+// straight into the next row's buckets (merge_expand), so the frontier is
+// never materialised in HBM.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -90,8 +90,8 @@ class Engine {
   std::mutex mu_;
   hipStream_t stream_ = nullptr;
   hipEvent_t ev_start_ = nullptr, ev_stop_ = nullptr;
-  void* region_[3] = {nullptr, nullptr, nullptr};
-  size_t region_bytes_[3] = {0, 0, 0};
+  void* region_[2] = {nullptr, nullptr};
+  size_t region_bytes_[2] = {0, 0};
   uint64_t* counters_ = nullptr;   // device, kNumCounters
   uint64_t* h_counters_ = nullptr; // pinned host mirror
   uint32_t* hist_ = nullptr;
@@ -100,8 +100,12 @@ class Engine {
   size_t offsets_cap_ = 0;
   uint64_t* segs_[2] = {nullptr, nullptr};
   size_t segs_cap_[2] = {0, 0};
-  uint64_t* slice_sums_ = nullptr;
-  size_t slice_cap_ = 0;
+  uint64_t* partial_ = nullptr;
+  size_t partial_cap_ = 0;
+  uint64_t* counters_rows_ = nullptr;  // per-row counters, (n + 1) * kNumCounters
+  size_t counters_rows_cap_ = 0;
+  uint64_t* seed_ = nullptr;           // seed frontier (SoA)
+  size_t seed_cap_ = 0;
   uint64_t budget_ = 0;
   Legs occupied_{};
 

@@ -86,7 +86,7 @@ int Engine::init(std::string* err) {
 }
 
 uint64_t Engine::arena_bytes() const {
-  return region_bytes_[0] + region_bytes_[1] + region_bytes_[2];
+  return region_bytes_[0] + region_bytes_[1];
 }
 
 int Engine::ensure_region(int idx, size_t bytes, std::string* err) {
@@ -217,6 +217,10 @@ int Engine::run(const RunSpec& spec, RunOut* out, std::string* err) {
     *err = "engine: slice ids do not fit above the 3N-bit boundary key";
     return 1;
   }
+  if (spec.first_row >= n || spec.seeds.empty()) {
+    *err = "engine: nothing to contract";
+    return 1;
+  }
   if (budget_ == 0 || (spec.budget_bytes && spec.budget_bytes != budget_)) {
     if (spec.budget_bytes) {
       budget_ = spec.budget_bytes;
@@ -227,14 +231,17 @@ int Engine::run(const RunSpec& spec, RunOut* out, std::string* err) {
     }
   }
   const size_t rb = record_bytes(L);
-  const size_t soa = 8 + 4 * (size_t)L;
+  const double R = 8.0 + 4.0 * L;  // algorithmic bytes per record: key + L residues
   Primes pr{};
   for (int l = 0; l < L; ++l) pr.p[l] = (uint32_t)spec.primes[l];
+  u128 fact;
+  factorial(n, &fact);
 
   RowGeom g{};
   g.n = n;
   g.board_mask = mask_bits(n);
   g.state_mask = mask_bits(3 * n);
+  g.orbit_mask = g.board_mask;
   g.col_in = occupied_.column_in;
   g.col_out = occupied_.column_out;
   g.dr_in = occupied_.diag_dr_in;
@@ -242,193 +249,222 @@ int Engine::run(const RunSpec& spec, RunOut* out, std::string* err) {
   g.dl_in = occupied_.diag_dl_in;
   g.dl_out = occupied_.diag_dl_out;
 
-  // Seed the frontier into region 0.
-  int fi = 0;
-  uint64_t S = spec.seeds.size();
-  uint64_t stride = std::max<uint64_t>(S, 1);
+  const uint32_t rbits = std::max<uint32_t>(1, std::min<uint32_t>(spec.radix_bits, kMaxDigitBits));
+  const uint64_t target = std::max<uint32_t>(256, std::min<uint32_t>(spec.tile_states, kMergeCap));
+  auto total_bits = [&](uint64_t C) -> unsigned {
+    return C <= target ? 0u : ceil_log2((C + target - 1) / target);
+  };
+
+  // Per-row counters, cleared once.
+  const size_t ncount = (size_t)(n + 1) * kNumCounters;
+  int rc = ensure_buffer(&counters_rows_, &counters_rows_cap_, ncount, err);
+  if (rc) return rc;
+  HIP_TRY(hipMemsetAsync(counters_rows_, 0, ncount * sizeof(uint64_t), stream_));
+  auto row_counters = [&](unsigned row) { return counters_rows_ + (size_t)row * kNumCounters; };
+
+  // ---- seed frontier (SoA) and the first row's expansion
+  const uint64_t S0 = spec.seeds.size();
+  const size_t seed_bytes = S0 * (8 + 4 * (size_t)L);
+  rc = ensure_buffer(&seed_, &seed_cap_, (seed_bytes + 7) / 8, err);
+  if (rc) return rc;
   {
-    int rc = ensure_region(fi, stride * soa, err);
-    if (rc) return rc;
-    std::vector<uint64_t> keys(S);
-    std::vector<uint32_t> res((size_t)L * stride, 0);
-    for (uint64_t i = 0; i < S; ++i) {
+    std::vector<uint64_t> host((seed_bytes + 7) / 8, 0);
+    uint64_t* keys = host.data();
+    uint32_t* res = (uint32_t*)(keys + S0);
+    for (uint64_t i = 0; i < S0; ++i) {
       keys[i] = spec.seeds[i].key;
-      for (int l = 0; l < L; ++l) res[(size_t)l * stride + i] = (uint32_t)(spec.seeds[i].weight % spec.primes[l]);
+      for (int l = 0; l < L; ++l) res[(size_t)l * S0 + i] = (uint32_t)(spec.seeds[i].weight % spec.primes[l]);
     }
-    char* base = (char*)region_[fi];
-    if (S) {
-      HIP_TRY(hipMemcpyAsync(base, keys.data(), S * 8, hipMemcpyHostToDevice, stream_));
-      HIP_TRY(hipMemcpyAsync(base + stride * 8, res.data(), (size_t)L * stride * 4,
-                             hipMemcpyHostToDevice, stream_));
-    }
+    HIP_TRY(hipMemcpyAsync(seed_, host.data(), seed_bytes, hipMemcpyHostToDevice, stream_));
   }
   out->layers.clear();
-  out->peak_states = std::max<uint64_t>(1, S);
+  out->peak_states = S0;
   out->total_candidates = out->total_matched = 0;
   out->device_peak_bytes = 0;
   out->residues.assign(L, 0);
-  u128 fact;
-  factorial(n, &fact);
   HIP_TRY(hipEventRecord(ev_start_, stream_));
 
+  const unsigned f = spec.first_row;
+  const bool orbit = spec.orbit_first_row && f == 0;
+  const uint64_t* skeys = seed_;
+  const uint32_t* sres = (const uint32_t*)(seed_ + S0);
+  uint64_t* c0 = row_counters(f);
+  if (orbit) {
+    LAUNCH("count", 8.0 * S0, launch_count(skeys, S0, g, c0 + kMatchedSlot, stream_));
+    g.orbit_row = 1;
+    g.orbit_mask = mask_bits((n - 1) / 2 + 1);  // columns c <= n-1-c
+    g.centre_bit = (n % 2) ? ((uint64_t)1 << (n / 2)) : 0;
+  }
+  LAUNCH("count", 8.0 * S0, launch_count(skeys, S0, g, c0, stream_));
+  HIP_TRY(hipMemcpyAsync(h_counters_, c0, sizeof(uint64_t) * kNumCounters, hipMemcpyDeviceToHost, stream_));
+  SYNC_AND_RESOLVE();
+  uint64_t C = h_counters_[kCandidates];  // candidates of the current row
+  u128 matched = orbit ? (u128)h_counters_[kMatchedSlot] : (u128)C;
+
+  int cur = 0;  // buffer holding the current row's final buckets
+  uint64_t nseg = 1;
+  int sidx = 0;
   std::vector<uint64_t> final_res(L, 0);
-  for (unsigned row = spec.first_row; row < n; ++row) {
-    const auto rt0 = Clock::now();
-    const bool orbit = spec.orbit_first_row && row == 0;
-    LayerOut lo{};
-    lo.row = row;
-    lo.input_states = S;
-    lo.candidates = (u128)n * S;
-    char* fbase = (char*)region_[fi];
-    const uint64_t* fkeys = (const uint64_t*)fbase;
-    const uint32_t* fres = (const uint32_t*)(fbase + stride * 8);
+  uint64_t S_in = S0;
+  unsigned levels = 0;
 
-    g.orbit_row = orbit ? 1u : 0u;
-    g.orbit_mask = g.board_mask;
-    g.centre_bit = 0;
-    uint64_t C = 0;
-    HIP_TRY(hipMemsetAsync(counters_, 0, sizeof(uint64_t) * kNumCounters, stream_));
-    if (S) {
-      if (orbit) {
-        LAUNCH("count", 8.0 * S, launch_count(fkeys, S, g, counters_ + kMatchedSlot, stream_));
-        g.orbit_mask = mask_bits((n - 1) / 2 + 1);  // columns c <= n-1-c
-        g.centre_bit = (n % 2) ? ((uint64_t)1 << (n / 2)) : 0;
-      }
-      LAUNCH("count", 8.0 * S, launch_count(fkeys, S, g, counters_, stream_));
-      HIP_TRY(hipMemcpyAsync(h_counters_, counters_, sizeof(uint64_t) * kNumCounters,
-                             hipMemcpyDeviceToHost, stream_));
-      SYNC_AND_RESOLVE();
-      C = h_counters_[kCandidates];
-      lo.matched = orbit ? (u128)h_counters_[kMatchedSlot] : (u128)C;
+  auto partition_to_final = [&](int from, unsigned d1, unsigned B, uint64_t Crow) -> int {
+    // level-1 buckets of `from` (segs_[sidx], 2^d1 of them) -> final buckets
+    cur = from;
+    nseg = (uint64_t)1 << d1;
+    levels = d1 ? 1 : 0;
+    unsigned used = d1;
+    while (used < B) {
+      const unsigned dk = std::min<unsigned>(rbits, B - used);
+      used += dk;
+      const int to = cur ^ 1;
+      int e = ensure_region(to, Crow * rb, err);
+      if (e) return e;
+      e = ensure_buffer(&segs_[sidx ^ 1], &segs_cap_[sidx ^ 1], (nseg << dk) + 1, err);
+      if (e) return e;
+      LAUNCH("partition", 2.0 * R * Crow,
+             launch_partition(L, region_[cur], region_[to], segs_[sidx], (uint32_t)nseg, 64 - used, dk,
+                              segs_[sidx ^ 1], stream_));
+      cur = to;
+      sidx ^= 1;
+      nseg <<= dk;
+      levels += 1;
     }
-    lo.completed = C;
-    if (S == 0 || C == 0) {
-      lo.output_states = 0;
-      lo.output_weight = 0;
-      S = 0;
-      lo.elapsed_s = std::chrono::duration<double>(Clock::now() - rt0).count();
-      lo.device_bytes = arena_bytes();
-      out->layers.push_back(lo);
-      out->total_candidates += lo.candidates;
-      out->total_matched += lo.matched;
-      std::fill(final_res.begin(), final_res.end(), 0);
-      continue;
-    }
+    return 0;
+  };
 
-    // Plan the hash-radix levels so final buckets hold ~tile_states records.
-    const uint32_t rbits = std::max<uint32_t>(1, std::min<uint32_t>(spec.radix_bits, 10));
-    const uint64_t target = std::max<uint32_t>(256, std::min<uint32_t>(spec.tile_states, 2560));
-    unsigned total_bits = C <= target ? 0 : ceil_log2((C + target - 1) / target);
-    std::vector<unsigned> digits;
-    if (total_bits) {
-      const unsigned levels = (total_bits + rbits - 1) / rbits;
-      unsigned left = total_bits;
-      for (unsigned k = 0; k < levels; ++k) {
-        const unsigned d = (left + (levels - k) - 1) / (levels - k);
-        digits.push_back(d);
-        left -= d;
-      }
-    } else {
-      digits.push_back(0);
-    }
-    if ((C >> digits[0]) > 0xffffffffull) {
-      *err = "row too large for a 32-bit level-1 bucket";
-      return 4;
-    }
-    const int pa = (fi + 1) % 3, pb = (fi + 2) % 3;
-    int rc = ensure_region(pa, C * rb, err);
+  if (C) {
+    const unsigned B = total_bits(C);
+    const unsigned d1 = std::min<unsigned>(B, kFineBits);
+    const uint32_t D1 = 1u << d1;
+    uint64_t tiles = std::min<uint64_t>(1024, std::max<uint64_t>(1, (S0 + 4095) / 4096));
+    const uint64_t per_tile = (S0 + tiles - 1) / tiles;
+    tiles = (S0 + per_tile - 1) / per_tile;
+    rc = ensure_region(0, C * rb, err);
     if (rc) return rc;
-    rc = ensure_region(pb, C * rb, err);
-    if (rc) return rc;
-
-    // Level 1 fused with the expansion.
-    const uint32_t D1 = 1u << digits[0];
-    uint64_t tiles = std::min<uint64_t>(1024, std::max<uint64_t>(1, (S + 4095) / 4096));
-    const uint64_t per_tile = (S + tiles - 1) / tiles;
-    tiles = (S + per_tile - 1) / per_tile;
     rc = ensure_buffer(&hist_, &hist_cap_, tiles * D1, err);
     if (rc) return rc;
     rc = ensure_buffer(&offsets_, &offsets_cap_, tiles * D1, err);
     if (rc) return rc;
-    uint64_t nseg_final = 1;
-    for (unsigned d : digits) nseg_final <<= d;
-    rc = ensure_buffer(&segs_[0], &segs_cap_[0], nseg_final + 1, err);
+    rc = ensure_buffer(&partial_, &partial_cap_, (tiles * D1) / 4096 + 2, err);
     if (rc) return rc;
-    rc = ensure_buffer(&segs_[1], &segs_cap_[1], nseg_final + 1, err);
+    rc = ensure_buffer(&segs_[0], &segs_cap_[0], (size_t)D1 + 1, err);
     if (rc) return rc;
+    sidx = 0;
+    LAUNCH("expand_hist", 8.0 * S0,
+           launch_expand_hist(skeys, S0, g, d1, (uint32_t)tiles, per_tile, hist_, stream_));
+    LAUNCH("scan", 12.0 * tiles * D1,
+           launch_scan(hist_, D1, (uint32_t)tiles, 1, offsets_, segs_[0], partial_, c0, stream_));
+    LAUNCH("expand_scatter", R * (S0 + C),
+           launch_expand_scatter(L, skeys, sres, S0, S0, g, pr, d1, (uint32_t)tiles, per_tile, offsets_,
+                                 region_[0], stream_));
+    rc = partition_to_final(0, d1, B, C);
+    if (rc) return rc;
+  }
+  g.orbit_row = 0;
+  g.orbit_mask = g.board_mask;
+  g.centre_bit = 0;
 
-    const double R = (double)soa;  // algorithmic bytes per record: key + L residues
-    LAUNCH("expand_hist", 8.0 * S,
-           launch_expand_hist(fkeys, S, g, digits[0], (uint32_t)tiles, per_tile, hist_, stream_));
-    LAUNCH("scan_tiles", 12.0 * tiles * D1,
-           launch_scan_tiles(hist_, (uint32_t)tiles, digits[0], offsets_, segs_[0], counters_, stream_));
-    LAUNCH("expand_scatter", R * (S + C),
-           launch_expand_scatter(L, fkeys, fres, S, stride, g, pr, digits[0], (uint32_t)tiles,
-                                 per_tile, offsets_, region_[pa], stream_));
-    int cur = pa, oth = pb, sidx = 0;
-    uint64_t nseg = D1;
-    unsigned used = digits[0];
-    for (size_t k = 1; k < digits.size(); ++k) {
-      used += digits[k];
-      LAUNCH("partition", 2.0 * R * C,
-             launch_partition(L, region_[cur], region_[oth], segs_[sidx], (uint32_t)nseg, 64 - used,
-                              digits[k], segs_[sidx ^ 1], stream_));
-      std::swap(cur, oth);
-      sidx ^= 1;
-      nseg <<= digits[k];
-    }
-    // Merge into the free partition buffer as the next SoA frontier.
-    char* obase = (char*)region_[oth];
+  for (unsigned row = f; row < n; ++row) {
+    const auto rt0 = Clock::now();
+    LayerOut lo{};
+    lo.row = row;
+    lo.input_states = S_in;
+    lo.candidates = (u128)n * S_in;
+    lo.matched = matched;
+    lo.completed = C;
+    uint64_t* cr = row_counters(row);
     const bool last = row + 1 == n;
-    LAUNCH("merge", R * C,
-           launch_merge(L, region_[cur], segs_[sidx], (uint32_t)nseg, pr, (uint64_t*)obase,
-                        (uint32_t*)(obase + C * 8), C, last ? g.board_mask : 0, counters_, 4096,
-                        stream_));
-    const int merge_pending = profiling_ ? (int)pending_.size() - 1 : -1;
-    HIP_TRY(hipMemcpyAsync(h_counters_, counters_, sizeof(uint64_t) * kNumCounters,
-                           hipMemcpyDeviceToHost, stream_));
-    HIP_TRY(hipStreamSynchronize(stream_));
-    // The merge's output size is known only now: add its written bytes.
-    if (merge_pending >= 0) pending_[merge_pending].bytes += R * (double)h_counters_[kOutStates];
-    if (!pending_.empty()) {
-      int rc2 = resolve_pending(err);
-      if (rc2) return rc2;
+    uint64_t C_next = 0;
+    if (C == 0) {
+      lo.output_states = 0;
+      lo.output_weight = 0;
+    } else {
+      const uint32_t W = (uint32_t)std::min<uint64_t>(nseg, last ? 1024 : 512);
+      if (last) {
+        LAUNCH("merge_last", R * C,
+               launch_merge_expand(kMergeLast, L, region_[cur], segs_[sidx], (uint32_t)nseg, W, g, pr, 0,
+                                   nullptr, nullptr, nullptr, g.board_mask, cr, stream_));
+      } else {
+        // COUNT: successor digits of the merged states at the fine resolution.
+        const uint32_t DF = 1u << kFineBits;
+        rc = ensure_buffer(&hist_, &hist_cap_, (size_t)DF * W, err);
+        if (rc) return rc;
+        rc = ensure_buffer(&offsets_, &offsets_cap_, (size_t)DF * W, err);
+        if (rc) return rc;
+        rc = ensure_buffer(&partial_, &partial_cap_, ((size_t)DF * W) / 4096 + 2, err);
+        if (rc) return rc;
+        const int nsidx = sidx ^ 1;
+        rc = ensure_buffer(&segs_[nsidx], &segs_cap_[nsidx], (size_t)DF + 1, err);
+        if (rc) return rc;
+        uint64_t* cn = row_counters(row + 1);
+        LAUNCH("merge_count", 8.0 * C,
+               launch_merge_expand(kMergeCount, L, region_[cur], segs_[sidx], (uint32_t)nseg, W, g, pr,
+                                   kFineBits, hist_, nullptr, nullptr, 0, cr, stream_));
+        LAUNCH("scan", 12.0 * DF * W,
+               launch_scan(hist_, DF, W, 1, offsets_, segs_[nsidx], partial_, cn, stream_));
+        HIP_TRY(hipMemcpyAsync(h_counters_, cn, sizeof(uint64_t) * kNumCounters, hipMemcpyDeviceToHost,
+                               stream_));
+        SYNC_AND_RESOLVE();
+        C_next = h_counters_[kCandidates];
+        if (C_next) {
+          const unsigned B = total_bits(C_next);
+          const unsigned d1 = std::min<unsigned>(B, kFineBits);
+          if (d1 < kFineBits) {
+            LAUNCH("scan", 12.0 * DF * W,
+                   launch_scan(hist_, 1u << d1, W, 1u << (kFineBits - d1), offsets_, segs_[nsidx], partial_,
+                               cn, stream_));
+          }
+          const int to = cur ^ 1;
+          rc = ensure_region(to, C_next * rb, err);
+          if (rc) return rc;
+          LAUNCH("merge_scatter", R * (C + C_next),
+                 launch_merge_expand(kMergeScatter, L, region_[cur], segs_[sidx], (uint32_t)nseg, W, g, pr, d1,
+                                     nullptr, offsets_, region_[to], 0, cr, stream_));
+          sidx = nsidx;
+          rc = partition_to_final(to, d1, B, C_next);
+          if (rc) return rc;
+        } else {
+          // No successor survives: merge for the statistics only.
+          LAUNCH("merge_last", R * C,
+                 launch_merge_expand(kMergeLast, L, region_[cur], segs_[sidx], (uint32_t)nseg, W, g, pr, 0,
+                                     nullptr, nullptr, nullptr, 0, cr, stream_));
+        }
+      }
+      HIP_TRY(hipMemcpyAsync(h_counters_, cr, sizeof(uint64_t) * kNumCounters, hipMemcpyDeviceToHost, stream_));
+      SYNC_AND_RESOLVE();
+      if (h_counters_[kError]) {
+        *err = "LDS merge table overflow in row " + std::to_string(row + 1);
+        return 4;
+      }
+      std::vector<uint64_t> w(L);
+      for (int l = 0; l < L; ++l) {
+        w[l] = h_counters_[kWeight0 + l] % spec.primes[l];
+        if (last) final_res[l] = h_counters_[kFinal0 + l] % spec.primes[l];
+      }
+      std::string cerr;
+      if (!crt_reconstruct(w.data(), spec.primes.data(), L, &lo.output_weight, &cerr)) {
+        *err = cerr;
+        return 2;
+      }
+      if (lo.output_weight > fact) {
+        *err = "CRT reconstruction exceeds certified N! count bound after row " + std::to_string(row + 1);
+        return 2;
+      }
+      lo.output_states = h_counters_[kOutStates];
     }
-    if (h_counters_[kError]) {
-      *err = "LDS merge table overflow in row " + std::to_string(row + 1);
-      return 4;
-    }
-    const uint64_t S_out = h_counters_[kOutStates];
-    std::vector<uint64_t> w(L);
-    for (int l = 0; l < L; ++l) {
-      w[l] = h_counters_[kWeight0 + l] % spec.primes[l];
-      final_res[l] = h_counters_[kFinal0 + l] % spec.primes[l];
-    }
-    u128 weight;
-    std::string cerr;
-    if (!crt_reconstruct(w.data(), spec.primes.data(), L, &weight, &cerr)) {
-      *err = cerr;
-      return 2;
-    }
-    if (weight > fact) {
-      *err = "CRT reconstruction exceeds certified N! count bound after row " + std::to_string(row + 1);
-      return 2;
-    }
-    lo.output_states = S_out;
-    lo.output_weight = weight;
-    lo.levels = (uint32_t)digits.size() - (digits[0] == 0 ? 1 : 0);
-    lo.buckets = (uint32_t)nseg;
-    lo.device_bytes = arena_bytes() + hist_cap_ * 4 + offsets_cap_ * 8 +
-                      (segs_cap_[0] + segs_cap_[1]) * 8;
+    lo.levels = C ? levels : 0;
+    lo.buckets = C ? (uint32_t)nseg : 0;
+    lo.device_bytes = arena_bytes() + hist_cap_ * 4 + offsets_cap_ * 8 + (segs_cap_[0] + segs_cap_[1]) * 8;
     lo.elapsed_s = std::chrono::duration<double>(Clock::now() - rt0).count();
     out->layers.push_back(lo);
     out->total_candidates += lo.candidates;
     out->total_matched += lo.matched;
-    out->peak_states = std::max<uint64_t>(out->peak_states, S_out);
+    out->peak_states = std::max<uint64_t>(out->peak_states, lo.output_states);
     out->device_peak_bytes = std::max<uint64_t>(out->device_peak_bytes, lo.device_bytes);
-    S = S_out;
-    stride = C;
-    fi = oth;
+    S_in = lo.output_states;
+    C = C_next;
+    matched = C_next;
   }
   HIP_TRY(hipEventRecord(ev_stop_, stream_));
   HIP_TRY(hipEventSynchronize(ev_stop_));
@@ -436,12 +472,13 @@ int Engine::run(const RunSpec& spec, RunOut* out, std::string* err) {
   HIP_TRY(hipEventElapsedTime(&ms, ev_start_, ev_stop_));
   out->kernel_s = ms / 1000.0;
 
-  // Final v1/v2 contraction: the merge of the last row summed the residues of
-  // the states whose columns are all set.
+  // Final v1/v2 contraction: the last merge summed the residues of the states
+  // whose columns are all set.
   u128 count = 0;
-  if (spec.first_row < n) {
+  const bool alive = !out->layers.empty() && out->layers.back().output_states > 0;
+  if (alive) {
     std::string cerr;
-    if (S && !crt_reconstruct(final_res.data(), spec.primes.data(), L, &count, &cerr)) {
+    if (!crt_reconstruct(final_res.data(), spec.primes.data(), L, &count, &cerr)) {
       *err = cerr;
       return 2;
     }
@@ -451,7 +488,7 @@ int Engine::run(const RunSpec& spec, RunOut* out, std::string* err) {
     }
   }
   out->count = count;
-  for (int l = 0; l < L; ++l) out->residues[l] = S ? final_res[l] : 0;
+  for (int l = 0; l < L; ++l) out->residues[l] = alive ? final_res[l] : 0;
   out->elapsed_s = std::chrono::duration<double>(Clock::now() - t0).count();
   return 0;
 }

@@ -1,15 +1,18 @@
 // kernels.h -- host-callable launchers for the gfx950 row-contraction kernels.
 //
-// One contracted row is:
-//   count      : successors per parent (popcount of the legal positions)
-//   expand     : parents -> successor records, hash-radix digit of level 1
-//                (histogram pass + scan + LDS-staged scatter)
-//   partition  : further hash-radix levels, one workgroup per segment
-//   merge      : per final bucket, LDS hash table with modular adds of the
-//                CRT residue lanes, compacted into the next frontier
 // Keys are the reference's packed boundary (columns | diag_dr << n |
 // diag_dl << 2n, src/lib.rs:357-363) in a u64, optionally with a slice id
-// above bit 3n.  Residues are u32 values modulo the CRT primes.
+// above bit 3n.  Weights are u32 residues modulo the CRT primes.
+//
+// Row pipeline (candidates of row r live in hash-radix buckets):
+//   expand (seed rows only): frontier -> candidates, level-1 buckets
+//   partition             : one more hash-radix level inside each bucket
+//   merge_expand COUNT    : per bucket, LDS-merge keys, count successors of
+//                           the merged states per next-row level-1 digit
+//   scan                  : digit-major exclusive scan -> write offsets
+//   merge_expand SCATTER  : merge with residues, layer statistics, write the
+//                           successors into the next row's level-1 buckets
+//   merge_expand LAST     : last row: merge, statistics, final v1/v2 sum
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -19,6 +22,10 @@ namespace peps {
 
 constexpr int kMaxLanes = 4;
 constexpr uint64_t kEmptyKey = ~0ull;
+constexpr uint32_t kFineBits = 10;       // histogram resolution of the COUNT pass
+constexpr uint32_t kMaxDigitBits = 11;   // widest partition digit
+constexpr uint32_t kMergeSlots = 4096;   // LDS table slots per merge bucket
+constexpr uint32_t kMergeCap = 2560;     // records per merge pass (5/8 load)
 
 // Geometry of one row step (uniform across the launch).
 struct RowGeom {
@@ -38,41 +45,50 @@ struct Primes {
 
 // Device-side counters of one row (u64 each).
 enum Counter : int {
-  kCandidates = 0,  // successors emitted (completed row terms)
-  kOutStates = 1,   // merged states written
-  kWeight0 = 2,     // .. kWeight0 + L - 1: sum of residues of merged states (mod p per block)
-  kFinal0 = 6,      // .. kFinal0 + L - 1: same restricted to columns == board mask
-  kError = 10,      // nonzero: an LDS merge table overflowed
+  kCandidates = 0,   // successors emitted (completed row terms of the next row)
+  kOutStates = 1,    // merged states of this row
+  kWeight0 = 2,      // .. kWeight0 + L - 1: residue sums of merged states (mod p per block)
+  kFinal0 = 6,       // .. kFinal0 + L - 1: same restricted to columns == board mask
+  kError = 10,       // nonzero: an LDS merge table overflowed
   kMatchedSlot = 11, // legal positions before the row-0 orbit filter
   kNumCounters = 12
 };
 
-// Record sizes for the AoS partition buffers: u64 key + L u32 residues.
+enum MergeMode : int { kMergeCount = 0, kMergeScatter = 1, kMergeLast = 2 };
+
 size_t record_bytes(int lanes);
 
-// All launchers run on `stream` and return hipSuccess or the launch error.
 hipError_t launch_count(const uint64_t* keys, uint64_t S, const RowGeom& g, uint64_t* counters,
                         hipStream_t stream);
 
+// Seed-row expansion: histogram ([D][tiles], digit-major) then scatter.
 hipError_t launch_expand_hist(const uint64_t* keys, uint64_t S, const RowGeom& g, uint32_t digit_bits,
                               uint32_t tiles, uint64_t per_tile, uint32_t* hist, hipStream_t stream);
-
-hipError_t launch_scan_tiles(const uint32_t* hist, uint32_t tiles, uint32_t digit_bits,
-                             uint64_t* offsets, uint64_t* segs, uint64_t* counters,
-                             hipStream_t stream);
-
 hipError_t launch_expand_scatter(int lanes, const uint64_t* keys, const uint32_t* res,
                                  uint64_t S, uint64_t res_stride, const RowGeom& g, const Primes& pr,
                                  uint32_t digit_bits, uint32_t tiles, uint64_t per_tile,
                                  const uint64_t* offsets, void* out, hipStream_t stream);
 
+// Exclusive scan of a digit-major histogram hist[Dfine][W] coarsened by
+// `group` fine digits per output digit: offsets[d*W + w] (u64), segs[0..D]
+// the digit boundaries (segs[D] = total), counters[kCandidates] = total.
+// `partial` needs ceil(D*W / 4096) + 1 u64 of scratch.
+hipError_t launch_scan(const uint32_t* hist, uint32_t D, uint32_t W, uint32_t group,
+                       uint64_t* offsets, uint64_t* segs, uint64_t* partial, uint64_t* counters,
+                       hipStream_t stream);
+
 hipError_t launch_partition(int lanes, const void* in, void* out, const uint64_t* segs,
                             uint32_t nseg, uint32_t shift, uint32_t digit_bits, uint64_t* subsegs,
                             hipStream_t stream);
 
-hipError_t launch_merge(int lanes, const void* in, const uint64_t* segs, uint32_t nseg,
-                        const Primes& pr, uint64_t* out_keys, uint32_t* out_res,
-                        uint64_t out_stride, uint64_t final_mask, uint64_t* counters,
-                        uint32_t table_slots, hipStream_t stream);
+// Fused merge of row r's buckets with the expansion into row r+1.
+// COUNT: hist[(1<<kFineBits)][W] of successor digits.  SCATTER: successors
+// written to out at offsets[(1<<digit_bits)][W].  LAST: no expansion;
+// final_mask selects the states summed into kFinal0.
+hipError_t launch_merge_expand(int mode, int lanes, const void* in, const uint64_t* segs,
+                               uint32_t nseg, uint32_t W, const RowGeom& g, const Primes& pr,
+                               uint32_t digit_bits, uint32_t* hist, const uint64_t* offsets,
+                               void* out, uint64_t final_mask, uint64_t* counters,
+                               hipStream_t stream);
 
 }  // namespace peps

@@ -8,8 +8,16 @@
 // CRT residues (the reference's checked u128 add, src/lib.rs:5354-5367).
 //
 // Merge strategy: a most-significant-digit radix partition on an invertible
-// 64-bit mix of the key (so equal keys land in the same bucket and bucket
-// sizes are uniform), finished per bucket by an LDS open-addressing table.
+// 64-bit mix of the key (so equal keys share a bucket and bucket sizes are
+// uniform), finished per bucket by an LDS open-addressing table.  The merge of
+// row r is fused with the expansion into row r+1, so the frontier itself is
+// never written to HBM: candidates go straight from one row's buckets to the
+// next row's buckets.
+//
+// Memory-level parallelism: each thread issues a batch of independent loads
+// before using any of them, and workgroup barriers wait only for LDS traffic
+// (lds_barrier), so global loads of the next batch stay in flight across
+// them.
 #include <hip/hip_runtime.h>
 
 #include "kernels.h"
@@ -21,14 +29,20 @@ namespace {
 constexpr int kBlock = 256;           // threads per workgroup (4 wave64s)
 constexpr int kWaves = kBlock / 64;
 constexpr int kStage = 2048;          // LDS-staged records per flush
-constexpr int kMaxDigitBits = 10;     // up to 1024 buckets per partition level
-constexpr int kMaxDigits = 1 << kMaxDigitBits;
+constexpr int kPerThread = kStage / kBlock;
 
 template <int L>
 struct alignas(8) Rec {
   uint64_t key;
   uint32_t r[L];
 };
+
+// Workgroup barrier that waits for LDS operations only.  Outstanding global
+// loads keep flying; their consumers are guarded by the compiler's own
+// vmcnt waits.
+__device__ __forceinline__ void lds_barrier() {
+  asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
 
 // MurmurHash3's 64-bit finalizer (public domain): a bijection on u64, so
 // partitioning by its bits never separates equal keys.
@@ -69,6 +83,10 @@ __device__ __forceinline__ uint64_t successor(uint64_t key, uint64_t b, const Ro
   return (key & ~g.state_mask) | ncols | (ndr << g.n) | (ndl << (2 * g.n));
 }
 
+__device__ __forceinline__ uint32_t top_digit(uint64_t key, uint32_t bits) {
+  return bits ? (uint32_t)(mix64(key) >> (64 - bits)) : 0u;
+}
+
 // Exclusive scan of one u32 per thread over the workgroup.
 __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* total,
                                                          uint32_t* wave_tot) {
@@ -80,7 +98,7 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* t
     if (lane >= o) x += y;
   }
   if (lane == 63) wave_tot[wave] = x;
-  __syncthreads();
+  lds_barrier();
   uint32_t before = 0, all = 0;
 #pragma unroll
   for (int w = 0; w < kWaves; ++w) {
@@ -88,7 +106,30 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* t
     before += (w < wave) ? t : 0;
     all += t;
   }
-  __syncthreads();
+  lds_barrier();
+  *total = all;
+  return before + x - v;
+}
+
+__device__ __forceinline__ uint64_t block_exclusive_scan64(uint64_t v, uint64_t* total,
+                                                           uint64_t* wave_tot) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  uint64_t x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    uint64_t y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) wave_tot[wave] = x;
+  lds_barrier();
+  uint64_t before = 0, all = 0;
+#pragma unroll
+  for (int w = 0; w < kWaves; ++w) {
+    uint64_t t = wave_tot[w];
+    before += (w < wave) ? t : 0;
+    all += t;
+  }
+  lds_barrier();
   *total = all;
   return before + x - v;
 }
@@ -98,15 +139,15 @@ __device__ __forceinline__ uint64_t block_sum_u64(uint64_t v, uint64_t* scratch)
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
   if (lane == 0) scratch[wave] = v;
-  __syncthreads();
+  lds_barrier();
   uint64_t s = 0;
 #pragma unroll
   for (int w = 0; w < kWaves; ++w) s += scratch[w];
-  __syncthreads();
+  lds_barrier();
   return s;
 }
 
-// Exclusive scan of counts[0..D) in LDS into starts[0..D) (D <= kMaxDigits).
+// Exclusive scan of counts[0..D) in LDS into starts[0..D).
 __device__ __forceinline__ void scan_digits(const uint32_t* counts, uint32_t* starts, uint32_t D,
                                             uint32_t* wave_tot) {
   const uint32_t per = (D + kBlock - 1) / kBlock;
@@ -122,26 +163,26 @@ __device__ __forceinline__ void scan_digits(const uint32_t* counts, uint32_t* st
       run += counts[lo + i];
     }
   }
-  __syncthreads();
+  lds_barrier();
 }
 
-// LDS staging area shared by the expand-scatter and partition kernels: records
-// are appended with a per-bucket rank, then written out grouped by bucket so
+// LDS staging area of the expand-scatter and partition kernels: records are
+// appended with a per-bucket rank, then written out grouped by bucket so
 // consecutive threads store consecutive addresses of a bucket run.
-template <int L>
+template <int L, int DMAX>
 struct Stage {
   Rec<L> rec[kStage];
   uint16_t bucket[kStage];
   uint16_t rank[kStage];
   uint16_t order[kStage];
-  uint32_t count[kMaxDigits];
-  uint32_t start[kMaxDigits];
-  uint64_t gpos[kMaxDigits];
+  uint32_t count[DMAX];
+  uint32_t start[DMAX];
+  uint64_t gpos[DMAX];
   uint32_t wave_tot[kWaves];
 };
 
-template <int L>
-__device__ __forceinline__ void stage_put(Stage<L>& st, uint32_t slot, const Rec<L>& r,
+template <int L, int DMAX>
+__device__ __forceinline__ void stage_put(Stage<L, DMAX>& st, uint32_t slot, const Rec<L>& r,
                                           uint32_t b) {
   st.rec[slot] = r;
   st.bucket[slot] = (uint16_t)b;
@@ -149,24 +190,24 @@ __device__ __forceinline__ void stage_put(Stage<L>& st, uint32_t slot, const Rec
 }
 
 // Write the `staged` records grouped by bucket, advance gpos, clear counts.
-template <int L>
-__device__ void stage_flush(Stage<L>& st, uint32_t staged, uint32_t D, Rec<L>* __restrict__ out) {
-  __syncthreads();
+template <int L, int DMAX>
+__device__ void stage_flush(Stage<L, DMAX>& st, uint32_t staged, uint32_t D, Rec<L>* __restrict__ out) {
+  lds_barrier();
   scan_digits(st.count, st.start, D, st.wave_tot);
   for (uint32_t i = threadIdx.x; i < staged; i += kBlock)
     st.order[st.start[st.bucket[i]] + st.rank[i]] = (uint16_t)i;
-  __syncthreads();
+  lds_barrier();
   for (uint32_t j = threadIdx.x; j < staged; j += kBlock) {
     const uint32_t i = st.order[j];
     const uint32_t b = st.bucket[i];
     out[st.gpos[b] + (j - st.start[b])] = st.rec[i];
   }
-  __syncthreads();
+  lds_barrier();
   for (uint32_t b = threadIdx.x; b < D; b += kBlock) {
     st.gpos[b] += st.count[b];
     st.count[b] = 0;
   }
-  __syncthreads();
+  lds_barrier();
 }
 
 // ------------------------------------------------------------------ count --
@@ -184,15 +225,16 @@ __global__ __launch_bounds__(kBlock) void k_count(const uint64_t* __restrict__ k
 
 // ------------------------------------------------------------ expand hist --
 
+constexpr uint32_t kExpandDigits = 1u << kFineBits;
+
 __global__ __launch_bounds__(kBlock) void k_expand_hist(const uint64_t* __restrict__ keys,
                                                         uint64_t S, RowGeom g, uint32_t digit_bits,
                                                         uint64_t per_tile,
                                                         uint32_t* __restrict__ hist) {
-  __shared__ uint32_t lh[kMaxDigits];
+  __shared__ uint32_t lh[kExpandDigits];
   const uint32_t D = 1u << digit_bits;
-  const uint32_t shift = 64 - digit_bits;
   for (uint32_t i = threadIdx.x; i < D; i += kBlock) lh[i] = 0;
-  __syncthreads();
+  lds_barrier();
   const uint64_t lo = (uint64_t)blockIdx.x * per_tile;
   const uint64_t hi = lo + per_tile < S ? lo + per_tile : S;
   for (uint64_t i = lo + threadIdx.x; i < hi; i += kBlock) {
@@ -201,57 +243,11 @@ __global__ __launch_bounds__(kBlock) void k_expand_hist(const uint64_t* __restri
     while (free) {
       const uint64_t b = free & (~free + 1);
       free ^= b;
-      const uint32_t d = digit_bits ? (uint32_t)(mix64(successor(key, b, g)) >> shift) : 0u;
-      atomicAdd(&lh[d], 1u);
+      atomicAdd(&lh[top_digit(successor(key, b, g), digit_bits)], 1u);
     }
   }
-  __syncthreads();
-  for (uint32_t i = threadIdx.x; i < D; i += kBlock) hist[(uint64_t)blockIdx.x * D + i] = lh[i];
-}
-
-// ------------------------------------------------------------- scan tiles --
-// One workgroup of 1024 threads.  hist is [tiles][D]; offsets (same layout)
-// receive each tile's first output position per digit in digit-major order;
-// segs[0..D] receive the digit bucket boundaries; counters[kCandidates] the total.
-
-constexpr int kScanBlock = 1024;
-
-__global__ __launch_bounds__(kScanBlock) void k_scan_tiles(const uint32_t* __restrict__ hist,
-                                                           uint32_t tiles, uint32_t digit_bits,
-                                                           uint64_t* __restrict__ offsets,
-                                                           uint64_t* __restrict__ segs,
-                                                           uint64_t* __restrict__ counters) {
-  __shared__ uint64_t part[kScanBlock];
-  __shared__ uint64_t dstart[kMaxDigits + 1];
-  const uint32_t D = 1u << digit_bits;
-  const uint32_t G = kScanBlock / D;  // tile groups
-  const uint32_t d = threadIdx.x % D, gi = threadIdx.x / D;
-  const uint32_t TG = (tiles + G - 1) / G;
-  const uint32_t t0 = gi * TG, t1 = (t0 + TG < tiles) ? t0 + TG : tiles;
-  uint64_t s = 0;
-  if (gi < G)
-    for (uint32_t t = t0; t < t1; ++t) s += hist[(uint64_t)t * D + d];
-  part[threadIdx.x] = s;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    uint64_t run = 0;
-    for (uint32_t dd = 0; dd < D; ++dd) {
-      dstart[dd] = run;
-      for (uint32_t gg = 0; gg < G; ++gg) run += part[gg * D + dd];
-    }
-    dstart[D] = run;
-  }
-  __syncthreads();
-  if (gi < G) {
-    uint64_t run = dstart[d];
-    for (uint32_t gg = 0; gg < gi; ++gg) run += part[gg * D + d];
-    for (uint32_t t = t0; t < t1; ++t) {
-      offsets[(uint64_t)t * D + d] = run;
-      run += hist[(uint64_t)t * D + d];
-    }
-  }
-  for (uint32_t i = threadIdx.x; i <= D; i += kScanBlock) segs[i] = dstart[i];
-  if (threadIdx.x == 0) counters[kCandidates] = dstart[D];
+  lds_barrier();
+  for (uint32_t i = threadIdx.x; i < D; i += kBlock) hist[(uint64_t)i * gridDim.x + blockIdx.x] = lh[i];
 }
 
 // --------------------------------------------------------- expand scatter --
@@ -261,14 +257,13 @@ __global__ __launch_bounds__(kBlock) void k_expand_scatter(
     const uint64_t* __restrict__ keys, const uint32_t* __restrict__ res, uint64_t S,
     uint64_t res_stride, RowGeom g, Primes pr, uint32_t digit_bits, uint64_t per_tile,
     const uint64_t* __restrict__ offsets, Rec<L>* __restrict__ out) {
-  __shared__ Stage<L> st;
+  __shared__ Stage<L, kExpandDigits> st;
   const uint32_t D = 1u << digit_bits;
-  const uint32_t shift = 64 - digit_bits;
   for (uint32_t b = threadIdx.x; b < D; b += kBlock) {
     st.count[b] = 0;
-    st.gpos[b] = offsets[(uint64_t)blockIdx.x * D + b];
+    st.gpos[b] = offsets[(uint64_t)b * gridDim.x + blockIdx.x];
   }
-  __syncthreads();
+  lds_barrier();
   const uint64_t lo = (uint64_t)blockIdx.x * per_tile;
   const uint64_t hi = lo + per_tile < S ? lo + per_tile : S;
   uint32_t staged = 0;
@@ -289,12 +284,11 @@ __global__ __launch_bounds__(kBlock) void k_expand_scatter(
     uint32_t win = 0;
     while (win < total) {
       if (staged == kStage) {
-        stage_flush<L>(st, staged, D, out);
+        stage_flush<L, kExpandDigits>(st, staged, D, out);
         staged = 0;
       }
       const uint32_t space = kStage - staged;
       const uint32_t wend = (total - win < space) ? total : win + space;
-      // this thread's successors with global round index in [win, wend)
       if (cnt && pos < wend && pos + cnt > win) {
         uint64_t f = free;
         for (uint32_t j = 0; j < cnt; ++j) {
@@ -307,22 +301,100 @@ __global__ __launch_bounds__(kBlock) void k_expand_scatter(
           const bool twice = g.orbit_row && b != g.centre_bit;
 #pragma unroll
           for (int l = 0; l < L; ++l) rec.r[l] = twice ? addmod(r[l], r[l], pr.p[l]) : r[l];
-          const uint32_t d = digit_bits ? (uint32_t)(mix64(rec.key) >> shift) : 0u;
-          stage_put<L>(st, staged + (idx - win), rec, d);
+          stage_put<L, kExpandDigits>(st, staged + (idx - win), rec, top_digit(rec.key, digit_bits));
         }
       }
       staged += wend - win;
       win = wend;
-      __syncthreads();
+      lds_barrier();
     }
   }
-  if (staged) stage_flush<L>(st, staged, D, out);
+  if (staged) stage_flush<L, kExpandDigits>(st, staged, D, out);
+}
+
+// ------------------------------------------------------------------- scan --
+
+constexpr int kScanItems = 16;
+constexpr int kScanTile = kBlock * kScanItems;
+
+__device__ __forceinline__ uint32_t hist_value(const uint32_t* __restrict__ h, uint64_t i,
+                                               uint32_t W, uint32_t group) {
+  if (group == 1) return h[i];
+  const uint64_t dc = i / W, w = i - dc * W;
+  uint32_t s = 0;
+  for (uint32_t k = 0; k < group; ++k) s += h[(dc * group + k) * W + w];
+  return s;
+}
+
+__global__ __launch_bounds__(kBlock) void k_scan_reduce(const uint32_t* __restrict__ h, uint64_t M,
+                                                        uint32_t W, uint32_t group,
+                                                        uint64_t* __restrict__ partial) {
+  __shared__ uint64_t scratch[kWaves];
+  const uint64_t base = (uint64_t)blockIdx.x * kScanTile;
+  uint64_t s = 0;
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    const uint64_t i = base + (uint64_t)k * kBlock + threadIdx.x;
+    if (i < M) s += hist_value(h, i, W, group);
+  }
+  s = block_sum_u64(s, scratch);
+  if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+// One workgroup: exclusive scan of the per-tile partials; total into
+// segs[D] and counters[kCandidates].
+__global__ __launch_bounds__(kBlock) void k_scan_partials(uint64_t* __restrict__ partial,
+                                                          uint32_t nb, uint64_t* __restrict__ segs,
+                                                          uint32_t D, uint64_t* __restrict__ counters) {
+  __shared__ uint64_t wave_tot[kWaves];
+  uint64_t carry = 0;
+  for (uint32_t base = 0; base < nb; base += kBlock) {
+    const uint32_t i = base + threadIdx.x;
+    const uint64_t v = i < nb ? partial[i] : 0;
+    uint64_t tot;
+    const uint64_t ex = block_exclusive_scan64(v, &tot, wave_tot);
+    if (i < nb) partial[i] = carry + ex;
+    carry += tot;
+  }
+  if (threadIdx.x == 0) {
+    segs[D] = carry;
+    counters[kCandidates] = carry;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_scan_apply(const uint32_t* __restrict__ h, uint64_t M,
+                                                       uint32_t W, uint32_t group,
+                                                       const uint64_t* __restrict__ partial,
+                                                       uint64_t* __restrict__ offsets,
+                                                       uint64_t* __restrict__ segs) {
+  __shared__ uint64_t wave_tot[kWaves];
+  const uint64_t base = (uint64_t)blockIdx.x * kScanTile + (uint64_t)threadIdx.x * kScanItems;
+  uint32_t v[kScanItems];
+  uint64_t local = 0;
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    v[k] = (base + k < M) ? hist_value(h, base + k, W, group) : 0u;
+    local += v[k];
+  }
+  uint64_t tot;
+  uint64_t run = partial[blockIdx.x] + block_exclusive_scan64(local, &tot, wave_tot);
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    const uint64_t i = base + k;
+    if (i < M) {
+      offsets[i] = run;
+      if (i % W == 0) segs[i / W] = run;
+    }
+    run += v[k];
+  }
 }
 
 // -------------------------------------------------------------- partition --
 // One workgroup per segment (grid-stride).  Digit = bits [shift, shift+D) of
 // the key mix.  subsegs[s*D + d] receives the start of digit d of segment s;
 // subsegs[nseg*D] the end of the last segment.
+
+constexpr uint32_t kPartDigits = 1u << kMaxDigitBits;
 
 template <int L>
 __global__ __launch_bounds__(kBlock) void k_partition(const Rec<L>* __restrict__ in,
@@ -331,154 +403,249 @@ __global__ __launch_bounds__(kBlock) void k_partition(const Rec<L>* __restrict__
                                                       uint32_t nseg, uint32_t shift,
                                                       uint32_t digit_bits,
                                                       uint64_t* __restrict__ subsegs) {
-  __shared__ Stage<L> st;
+  __shared__ Stage<L, kPartDigits> st;
   const uint32_t D = 1u << digit_bits;
   const uint64_t dmask = D - 1;
   for (uint32_t s = blockIdx.x; s < nseg; s += gridDim.x) {
     const uint64_t lo = segs[s], hi = segs[s + 1];
     for (uint32_t b = threadIdx.x; b < D; b += kBlock) st.count[b] = 0;
-    __syncthreads();
-    for (uint64_t i = lo + threadIdx.x; i < hi; i += kBlock)
-      atomicAdd(&st.count[(uint32_t)((mix64(in[i].key) >> shift) & dmask)], 1u);
-    __syncthreads();
+    lds_barrier();
+    // Sweep 1: digit counts from the keys, kPerThread loads in flight per thread.
+    for (uint64_t base = lo; base < hi; base += kStage) {
+      uint64_t k[kPerThread];
+#pragma unroll
+      for (int j = 0; j < kPerThread; ++j) {
+        const uint64_t i = base + (uint64_t)j * kBlock + threadIdx.x;
+        k[j] = i < hi ? in[i].key : kEmptyKey;
+      }
+#pragma unroll
+      for (int j = 0; j < kPerThread; ++j)
+        if (k[j] != kEmptyKey) atomicAdd(&st.count[(uint32_t)((mix64(k[j]) >> shift) & dmask)], 1u);
+    }
+    lds_barrier();
     scan_digits(st.count, st.start, D, st.wave_tot);
     for (uint32_t b = threadIdx.x; b < D; b += kBlock) {
       st.gpos[b] = lo + st.start[b];
       subsegs[(uint64_t)s * D + b] = lo + st.start[b];
       st.count[b] = 0;
     }
-    __syncthreads();
+    lds_barrier();
+    // Sweep 2: the next chunk is loaded while the current one is flushed.
+    Rec<L> nx[kPerThread];
+#pragma unroll
+    for (int j = 0; j < kPerThread; ++j) {
+      const uint64_t i = lo + (uint64_t)j * kBlock + threadIdx.x;
+      if (i < hi) nx[j] = in[i];
+    }
     for (uint64_t base = lo; base < hi; base += kStage) {
-      const uint32_t n = (hi - base < (uint64_t)kStage) ? (uint32_t)(hi - base) : (uint32_t)kStage;
-      for (uint32_t j = threadIdx.x; j < n; j += kBlock) {
-        const Rec<L> rec = in[base + j];
-        stage_put<L>(st, j, rec, (uint32_t)((mix64(rec.key) >> shift) & dmask));
+      const uint32_t nrec = (hi - base < (uint64_t)kStage) ? (uint32_t)(hi - base) : (uint32_t)kStage;
+      Rec<L> cur[kPerThread];
+#pragma unroll
+      for (int j = 0; j < kPerThread; ++j) cur[j] = nx[j];
+      const uint64_t nb = base + kStage;
+      if (nb < hi) {
+#pragma unroll
+        for (int j = 0; j < kPerThread; ++j) {
+          const uint64_t i = nb + (uint64_t)j * kBlock + threadIdx.x;
+          if (i < hi) nx[j] = in[i];
+        }
       }
-      stage_flush<L>(st, n, D, out);
+#pragma unroll
+      for (int j = 0; j < kPerThread; ++j) {
+        const uint32_t slot = (uint32_t)j * kBlock + threadIdx.x;
+        if (slot < nrec) stage_put<L, kPartDigits>(st, slot, cur[j], (uint32_t)((mix64(cur[j].key) >> shift) & dmask));
+      }
+      stage_flush<L, kPartDigits>(st, nrec, D, out);
     }
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) subsegs[(uint64_t)nseg * D] = segs[nseg];
 }
 
-// ------------------------------------------------------------------ merge --
-// One workgroup per final bucket (grid-stride).  Records are inserted into an
-// LDS open-addressing table (linear probing on the mix's low bits) and the
-// residues of equal keys added modulo each prime.  A bucket larger than the
-// table's safe load is processed in sub-passes that each take the keys whose
-// mix falls in one residue class, so every pass sees disjoint keys.
+// ----------------------------------------------------------- merge-expand --
+// Workgroup w owns the buckets [w*per, (w+1)*per) of row r; the COUNT and
+// SCATTER launches use the same assignment, so they see identical merged
+// states.  Each bucket's records are inserted into an LDS open-addressing
+// table (linear probing on the mix's low bits); residues of equal keys are
+// added modulo each prime.  A bucket above the table's safe load is merged in
+// passes that each take the keys whose mix falls in one residue class.  The
+// merged states are then counted (layer statistics) and expanded through the
+// next row; the successors' digits index the next row's level-1 buckets.
 
-template <int L, int T>
-__global__ __launch_bounds__(kBlock) void k_merge(const Rec<L>* __restrict__ in,
-                                                  const uint64_t* __restrict__ segs, uint32_t nseg,
-                                                  Primes pr, uint64_t* __restrict__ out_keys,
-                                                  uint32_t* __restrict__ out_res,
-                                                  uint64_t out_stride, uint64_t final_mask,
-                                                  uint64_t* __restrict__ counters) {
+constexpr int kMergeBatch = kMergeCap / kBlock;  // records per thread per load batch
+
+template <int L, int MODE>
+__global__ __launch_bounds__(kBlock) void k_merge_expand(
+    const Rec<L>* __restrict__ in, const uint64_t* __restrict__ segs, uint32_t nseg, uint32_t per,
+    RowGeom g, Primes pr, uint32_t digit_bits, uint32_t* __restrict__ hist,
+    const uint64_t* __restrict__ offsets, Rec<L>* __restrict__ out, uint64_t final_mask,
+    uint64_t* __restrict__ counters) {
+  constexpr uint32_t T = kMergeSlots;
+  constexpr int LL = MODE == kMergeCount ? 1 : L;  // COUNT needs no residues
+  constexpr uint32_t DM = MODE == kMergeLast ? 1 : (1u << kFineBits);
   __shared__ uint64_t tkey[T];
-  __shared__ uint32_t tres[L][T];
-  __shared__ uint32_t rcount[T / kBlock][kWaves];
-  __shared__ uint64_t base_sh;
+  __shared__ uint32_t tres[LL][T];
+  __shared__ uint32_t lcount[DM];
+  __shared__ uint64_t gpos[MODE == kMergeScatter ? DM : 1];
   __shared__ uint64_t scratch[kWaves];
-  constexpr uint32_t kCap = (T * 5) / 8;
-  constexpr uint32_t kRounds = T / kBlock;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint32_t D = MODE == kMergeLast ? 1u : (1u << digit_bits);
+  const uint32_t W = gridDim.x;
+
+  for (uint32_t i = threadIdx.x; i < T; i += kBlock) {
+    tkey[i] = kEmptyKey;
+#pragma unroll
+    for (int l = 0; l < LL; ++l) tres[l][i] = 0;
+  }
+  if (MODE != kMergeLast) {
+    for (uint32_t d = threadIdx.x; d < D; d += kBlock) {
+      lcount[d] = 0;
+      if (MODE == kMergeScatter) gpos[d] = offsets[(uint64_t)d * W + blockIdx.x];
+    }
+  }
+  lds_barrier();
+
+  uint64_t uniq = 0;
   uint64_t wsum[L], fsum[L];
 #pragma unroll
   for (int l = 0; l < L; ++l) wsum[l] = fsum[l] = 0;
   uint32_t overflow = 0;
 
-  for (uint32_t s = blockIdx.x; s < nseg; s += gridDim.x) {
-    const uint64_t lo = segs[s], hi = segs[s + 1];
-    if (lo == hi) continue;
+  const uint32_t b0 = blockIdx.x * per;
+  const uint32_t b1 = (b0 + per < nseg) ? b0 + per : nseg;
+
+  // Prefetch of the first bucket (when it fits one batch).
+  Rec<L> buf[kMergeBatch];
+  auto load_batch = [&](uint64_t base, uint64_t hi) {
+#pragma unroll
+    for (int k = 0; k < kMergeBatch; ++k) {
+      const uint64_t i = base + (uint64_t)k * kBlock + threadIdx.x;
+      if (i < hi) {
+        if (MODE == kMergeCount)
+          buf[k].key = in[i].key;
+        else
+          buf[k] = in[i];
+      }
+    }
+  };
+  uint64_t plo = 0, phi = 0;
+  if (b0 < b1) {
+    plo = segs[b0];
+    phi = segs[b0 + 1];
+    load_batch(plo, phi);
+  }
+
+  for (uint32_t b = b0; b < b1; ++b) {
+    const uint64_t lo = plo, hi = phi;
     const uint64_t m = hi - lo;
-    const uint32_t passes = m <= kCap ? 1u : (uint32_t)((m + kCap - 1) / kCap);
+    const uint32_t passes = m <= kMergeCap ? 1u : (uint32_t)((m + kMergeCap - 1) / kMergeCap);
+    uint64_t nlo = 0, nhi = 0;
+    if (b + 1 < b1) {
+      nlo = segs[b + 1];
+      nhi = segs[b + 2];
+    }
     for (uint32_t pass = 0; pass < passes; ++pass) {
-      for (uint32_t i = threadIdx.x; i < T; i += kBlock) {
-        tkey[i] = kEmptyKey;
+      // ---- insert
+      for (uint64_t base = lo; base < hi; base += kMergeCap) {
+        const bool prefetched = (passes == 1);  // single-batch bucket already in buf
+        if (!prefetched) load_batch(base, hi);
 #pragma unroll
-        for (int l = 0; l < L; ++l) tres[l][i] = 0;
-      }
-      __syncthreads();
-      for (uint64_t i = lo + threadIdx.x; i < hi; i += kBlock) {
-        const Rec<L> rec = in[i];
-        const uint64_t h = mix64(rec.key);
-        if (passes > 1 && (uint32_t)((((h >> 24) & 0xffffffull) * passes) >> 24) != pass) continue;
-        uint32_t slot = (uint32_t)h & (T - 1);
-        bool placed = false;
-        for (uint32_t probe = 0; probe < T; ++probe) {
-          const unsigned long long old =
-              atomicCAS((unsigned long long*)&tkey[slot], (unsigned long long)kEmptyKey,
-                        (unsigned long long)rec.key);
-          if (old == kEmptyKey || old == rec.key) {
-            placed = true;
-            break;
+        for (int k = 0; k < kMergeBatch; ++k) {
+          const uint64_t i = base + (uint64_t)k * kBlock + threadIdx.x;
+          if (i >= hi) continue;
+          const uint64_t key = buf[k].key;
+          const uint64_t h = mix64(key);
+          if (passes > 1 && (uint32_t)((((h >> 12) & 0xffffffull) * passes) >> 24) != pass) continue;
+          uint32_t slot = (uint32_t)h & (T - 1);
+          bool placed = false;
+          for (uint32_t probe = 0; probe < T; ++probe) {
+            const unsigned long long old = atomicCAS((unsigned long long*)&tkey[slot],
+                                                     (unsigned long long)kEmptyKey,
+                                                     (unsigned long long)key);
+            if (old == kEmptyKey || old == key) {
+              placed = true;
+              break;
+            }
+            slot = (slot + 1) & (T - 1);
           }
-          slot = (slot + 1) & (T - 1);
-        }
-        if (!placed) {
-          overflow = 1;
-          continue;
-        }
+          if (!placed) {
+            overflow = 1;
+            continue;
+          }
+          if (MODE != kMergeCount) {
 #pragma unroll
-        for (int l = 0; l < L; ++l) {
-          uint32_t* a = &tres[l][slot];
-          uint32_t old = *a, assumed;
-          do {
-            assumed = old;
-            old = atomicCAS(a, assumed, addmod(assumed, rec.r[l], pr.p[l]));
-          } while (old != assumed);
+            for (int l = 0; l < L; ++l) {
+              uint32_t* a = &tres[l < LL ? l : 0][slot];
+              uint32_t old = *a, assumed;
+              do {
+                assumed = old;
+                old = atomicCAS(a, assumed, addmod(assumed, buf[k].r[l], pr.p[l]));
+              } while (old != assumed);
+            }
+          }
         }
       }
-      __syncthreads();
-      // Compact occupied slots: ballot ranks within each (round, wave).
-      for (uint32_t rd = 0; rd < kRounds; ++rd) {
-        const bool occ = tkey[rd * kBlock + threadIdx.x] != kEmptyKey;
-        const uint64_t bal = __ballot(occ);
-        if (lane == 0) rcount[rd][wave] = (uint32_t)__popcll(bal);
-      }
-      __syncthreads();
-      if (threadIdx.x == 0) {
-        uint32_t run = 0;
-        for (uint32_t rd = 0; rd < kRounds; ++rd)
-          for (int w = 0; w < kWaves; ++w) {
-            const uint32_t c = rcount[rd][w];
-            rcount[rd][w] = run;
-            run += c;
-          }
-        base_sh = run ? atomicAdd((unsigned long long*)&counters[kOutStates], (unsigned long long)run) : 0;
-      }
-      __syncthreads();
-      const uint64_t base = base_sh;
-      for (uint32_t rd = 0; rd < kRounds; ++rd) {
-        const uint32_t slot = rd * kBlock + threadIdx.x;
-        const uint64_t key = tkey[slot];
-        const bool occ = key != kEmptyKey;
-        const uint64_t bal = __ballot(occ);
-        if (occ) {
-          const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32),
-                                                         __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
-          const uint64_t pos = base + rcount[rd][wave] + rank;
-          out_keys[pos] = key;
-          const bool fin = final_mask && (key & final_mask) == final_mask;
+      lds_barrier();
+      // Prefetch the next bucket while this one is expanded.
+      if (pass + 1 == passes && b + 1 < b1 && nhi - nlo <= kMergeCap) load_batch(nlo, nhi);
+      // ---- statistics, expansion, reset
+      for (uint32_t s = threadIdx.x; s < T; s += kBlock) {
+        const uint64_t key = tkey[s];
+        if (key == kEmptyKey) continue;
+        uniq += 1;
+        uint32_t v[L];
+        if (MODE != kMergeCount) {
 #pragma unroll
           for (int l = 0; l < L; ++l) {
-            const uint32_t v = tres[l][slot];
-            out_res[(uint64_t)l * out_stride + pos] = v;
-            wsum[l] += v;
-            if (fin) fsum[l] += v;
+            v[l] = tres[l < LL ? l : 0][s];
+            wsum[l] += v[l];
+            tres[l < LL ? l : 0][s] = 0;
+          }
+          if (MODE == kMergeLast && (key & final_mask) == final_mask) {
+#pragma unroll
+            for (int l = 0; l < L; ++l) fsum[l] += v[l];
           }
         }
-      }
-      __syncthreads();
-    }
-  }
+        if (MODE != kMergeLast) {
+          uint64_t free = legal_positions(key, g);
+          while (free) {
+            const uint64_t bit = free & (~free + 1);
+            free ^= bit;
+            const uint64_t sk = successor(key, bit, g);
+            const uint32_t d = top_digit(sk, digit_bits);
+            const uint32_t rank = atomicAdd(&lcount[d], 1u);
+            if (MODE == kMergeScatter) {
+              Rec<L> rec;
+              rec.key = sk;
 #pragma unroll
-  for (int l = 0; l < L; ++l) {
-    const uint64_t w = block_sum_u64(wsum[l] % pr.p[l], scratch);
-    const uint64_t f = block_sum_u64(fsum[l] % pr.p[l], scratch);
-    if (threadIdx.x == 0) {
-      if (w) atomicAdd((unsigned long long*)&counters[kWeight0 + l], (unsigned long long)(w % pr.p[l]));
-      if (f) atomicAdd((unsigned long long*)&counters[kFinal0 + l], (unsigned long long)(f % pr.p[l]));
+              for (int l = 0; l < L; ++l) rec.r[l] = v[l];
+              out[gpos[d] + rank] = rec;
+            }
+          }
+        }
+        tkey[s] = kEmptyKey;
+      }
+      lds_barrier();
+    }
+    if (b + 1 < b1 && nhi - nlo > kMergeCap) {
+      // multi-pass bucket: loaded batch by batch inside its passes
+    }
+    plo = nlo;
+    phi = nhi;
+  }
+
+  // ---- flush statistics
+  if (MODE == kMergeCount) {
+    for (uint32_t d = threadIdx.x; d < D; d += kBlock) hist[(uint64_t)d * W + blockIdx.x] = lcount[d];
+  } else {
+    const uint64_t u = block_sum_u64(uniq, scratch);
+    if (threadIdx.x == 0 && u) atomicAdd((unsigned long long*)&counters[kOutStates], (unsigned long long)u);
+#pragma unroll
+    for (int l = 0; l < L; ++l) {
+      const uint64_t w = block_sum_u64(wsum[l] % pr.p[l], scratch);
+      if (threadIdx.x == 0 && w) atomicAdd((unsigned long long*)&counters[kWeight0 + l], (unsigned long long)(w % pr.p[l]));
+      if (MODE == kMergeLast) {
+        const uint64_t f = block_sum_u64(fsum[l] % pr.p[l], scratch);
+        if (threadIdx.x == 0 && f) atomicAdd((unsigned long long*)&counters[kFinal0 + l], (unsigned long long)(f % pr.p[l]));
+      }
     }
   }
   if (overflow) atomicOr((unsigned long long*)&counters[kError], 1ull);
@@ -512,18 +679,9 @@ hipError_t launch_count(const uint64_t* keys, uint64_t S, const RowGeom& g, uint
 hipError_t launch_expand_hist(const uint64_t* keys, uint64_t S, const RowGeom& g,
                               uint32_t digit_bits, uint32_t tiles, uint64_t per_tile,
                               uint32_t* hist, hipStream_t stream) {
-  if (digit_bits > kMaxDigitBits || tiles == 0) return hipErrorInvalidValue;
+  if (digit_bits > kFineBits || tiles == 0) return hipErrorInvalidValue;
   hipLaunchKernelGGL(k_expand_hist, dim3(tiles), dim3(kBlock), 0, stream, keys, S, g, digit_bits,
                      per_tile, hist);
-  return hipGetLastError();
-}
-
-hipError_t launch_scan_tiles(const uint32_t* hist, uint32_t tiles, uint32_t digit_bits,
-                             uint64_t* offsets, uint64_t* segs, uint64_t* counters,
-                             hipStream_t stream) {
-  if (digit_bits > kMaxDigitBits || tiles == 0) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(kScanBlock), 0, stream, hist, tiles, digit_bits,
-                     offsets, segs, counters);
   return hipGetLastError();
 }
 
@@ -531,7 +689,7 @@ hipError_t launch_expand_scatter(int lanes, const uint64_t* keys, const uint32_t
                                  uint64_t res_stride, const RowGeom& g, const Primes& pr,
                                  uint32_t digit_bits, uint32_t tiles, uint64_t per_tile,
                                  const uint64_t* offsets, void* out, hipStream_t stream) {
-  if (digit_bits > kMaxDigitBits || tiles == 0) return hipErrorInvalidValue;
+  if (digit_bits > kFineBits || tiles == 0) return hipErrorInvalidValue;
   dim3 grid(tiles), block(kBlock);
   switch (lanes) {
     case 1: hipLaunchKernelGGL(k_expand_scatter<1>, grid, block, 0, stream, keys, res, S, res_stride, g, pr, digit_bits, per_tile, offsets, (Rec<1>*)out); break;
@@ -543,11 +701,24 @@ hipError_t launch_expand_scatter(int lanes, const uint64_t* keys, const uint32_t
   return hipGetLastError();
 }
 
+hipError_t launch_scan(const uint32_t* hist, uint32_t D, uint32_t W, uint32_t group,
+                       uint64_t* offsets, uint64_t* segs, uint64_t* partial, uint64_t* counters,
+                       hipStream_t stream) {
+  const uint64_t M = (uint64_t)D * W;
+  if (M == 0 || group == 0) return hipErrorInvalidValue;
+  const uint32_t nb = (uint32_t)((M + kScanTile - 1) / kScanTile);
+  hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(kBlock), 0, stream, hist, M, W, group, partial);
+  hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(kBlock), 0, stream, partial, nb, segs, D, counters);
+  hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(kBlock), 0, stream, hist, M, W, group,
+                     (const uint64_t*)partial, offsets, segs);
+  return hipGetLastError();
+}
+
 hipError_t launch_partition(int lanes, const void* in, void* out, const uint64_t* segs,
                             uint32_t nseg, uint32_t shift, uint32_t digit_bits, uint64_t* subsegs,
                             hipStream_t stream) {
   if (digit_bits == 0 || digit_bits > kMaxDigitBits || nseg == 0) return hipErrorInvalidValue;
-  dim3 grid(nseg < 4096 ? nseg : 4096), block(kBlock);
+  dim3 grid(nseg < 8192 ? nseg : 8192), block(kBlock);
   switch (lanes) {
     case 1: hipLaunchKernelGGL(k_partition<1>, grid, block, 0, stream, (const Rec<1>*)in, (Rec<1>*)out, segs, nseg, shift, digit_bits, subsegs); break;
     case 2: hipLaunchKernelGGL(k_partition<2>, grid, block, 0, stream, (const Rec<2>*)in, (Rec<2>*)out, segs, nseg, shift, digit_bits, subsegs); break;
@@ -558,21 +729,38 @@ hipError_t launch_partition(int lanes, const void* in, void* out, const uint64_t
   return hipGetLastError();
 }
 
-hipError_t launch_merge(int lanes, const void* in, const uint64_t* segs, uint32_t nseg,
-                        const Primes& pr, uint64_t* out_keys, uint32_t* out_res,
-                        uint64_t out_stride, uint64_t final_mask, uint64_t* counters,
-                        uint32_t table_slots, hipStream_t stream) {
-  if (nseg == 0) return hipSuccess;
-  if (table_slots != 4096) return hipErrorInvalidValue;
-  dim3 grid(nseg < 8192 ? nseg : 8192), block(kBlock);
-  switch (lanes) {
-    case 1: hipLaunchKernelGGL((k_merge<1, 4096>), grid, block, 0, stream, (const Rec<1>*)in, segs, nseg, pr, out_keys, out_res, out_stride, final_mask, counters); break;
-    case 2: hipLaunchKernelGGL((k_merge<2, 4096>), grid, block, 0, stream, (const Rec<2>*)in, segs, nseg, pr, out_keys, out_res, out_stride, final_mask, counters); break;
-    case 3: hipLaunchKernelGGL((k_merge<3, 4096>), grid, block, 0, stream, (const Rec<3>*)in, segs, nseg, pr, out_keys, out_res, out_stride, final_mask, counters); break;
-    case 4: hipLaunchKernelGGL((k_merge<4, 4096>), grid, block, 0, stream, (const Rec<4>*)in, segs, nseg, pr, out_keys, out_res, out_stride, final_mask, counters); break;
+template <int L>
+static hipError_t launch_me(int mode, dim3 grid, hipStream_t stream, const void* in,
+                            const uint64_t* segs, uint32_t nseg, uint32_t per, const RowGeom& g,
+                            const Primes& pr, uint32_t digit_bits, uint32_t* hist,
+                            const uint64_t* offsets, void* out, uint64_t final_mask,
+                            uint64_t* counters) {
+  const Rec<L>* i = (const Rec<L>*)in;
+  Rec<L>* o = (Rec<L>*)out;
+  switch (mode) {
+    case kMergeCount: hipLaunchKernelGGL((k_merge_expand<L, kMergeCount>), grid, dim3(kBlock), 0, stream, i, segs, nseg, per, g, pr, digit_bits, hist, offsets, o, final_mask, counters); break;
+    case kMergeScatter: hipLaunchKernelGGL((k_merge_expand<L, kMergeScatter>), grid, dim3(kBlock), 0, stream, i, segs, nseg, per, g, pr, digit_bits, hist, offsets, o, final_mask, counters); break;
+    case kMergeLast: hipLaunchKernelGGL((k_merge_expand<L, kMergeLast>), grid, dim3(kBlock), 0, stream, i, segs, nseg, per, g, pr, digit_bits, hist, offsets, o, final_mask, counters); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
+}
+
+hipError_t launch_merge_expand(int mode, int lanes, const void* in, const uint64_t* segs,
+                               uint32_t nseg, uint32_t W, const RowGeom& g, const Primes& pr,
+                               uint32_t digit_bits, uint32_t* hist, const uint64_t* offsets,
+                               void* out, uint64_t final_mask, uint64_t* counters,
+                               hipStream_t stream) {
+  if (nseg == 0 || W == 0 || digit_bits > kFineBits) return hipErrorInvalidValue;
+  const uint32_t per = (nseg + W - 1) / W;
+  dim3 grid(W);
+  switch (lanes) {
+    case 1: return launch_me<1>(mode, grid, stream, in, segs, nseg, per, g, pr, digit_bits, hist, offsets, out, final_mask, counters);
+    case 2: return launch_me<2>(mode, grid, stream, in, segs, nseg, per, g, pr, digit_bits, hist, offsets, out, final_mask, counters);
+    case 3: return launch_me<3>(mode, grid, stream, in, segs, nseg, per, g, pr, digit_bits, hist, offsets, out, final_mask, counters);
+    case 4: return launch_me<4>(mode, grid, stream, in, segs, nseg, per, g, pr, digit_bits, hist, offsets, out, final_mask, counters);
+    default: return hipErrorInvalidValue;
+  }
 }
 
 }  // namespace peps
