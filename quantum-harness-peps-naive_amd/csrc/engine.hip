@@ -250,7 +250,10 @@ int Engine::run(const RunSpec& spec, RunOut* out, std::string* err) {
   g.dl_out = occupied_.diag_dl_out;
 
   const uint32_t rbits = std::max<uint32_t>(1, std::min<uint32_t>(spec.radix_bits, kMaxDigitBits));
-  const uint64_t target = std::max<uint32_t>(256, std::min<uint32_t>(spec.tile_states, kMergeCap));
+  // LDS merge table: 2048 slots (more workgroups per CU) for small tiles,
+  // 4096 slots otherwise; the bucket target stays within one merge pass.
+  const uint32_t table_slots = spec.tile_states <= 1024 ? 2048u : 4096u;
+  const uint64_t target = std::max<uint32_t>(256, std::min<uint32_t>(spec.tile_states, merge_cap(table_slots)));
   auto total_bits = [&](uint64_t C) -> unsigned {
     return C <= target ? 0u : ceil_log2((C + target - 1) / target);
   };
@@ -380,11 +383,11 @@ int Engine::run(const RunSpec& spec, RunOut* out, std::string* err) {
       lo.output_states = 0;
       lo.output_weight = 0;
     } else {
-      const uint32_t W = (uint32_t)std::min<uint64_t>(nseg, last ? 1024 : 512);
+      const uint32_t W = (uint32_t)std::min<uint64_t>(nseg, 1024);
       if (last) {
         LAUNCH("merge_last", R * C,
                launch_merge_expand(kMergeLast, L, region_[cur], segs_[sidx], (uint32_t)nseg, W, g, pr, 0,
-                                   nullptr, nullptr, nullptr, g.board_mask, cr, stream_));
+                                   nullptr, nullptr, nullptr, g.board_mask, cr, table_slots, stream_));
       } else {
         // COUNT: successor digits of the merged states at the fine resolution.
         const uint32_t DF = 1u << kFineBits;
@@ -400,7 +403,7 @@ int Engine::run(const RunSpec& spec, RunOut* out, std::string* err) {
         uint64_t* cn = row_counters(row + 1);
         LAUNCH("merge_count", 8.0 * C,
                launch_merge_expand(kMergeCount, L, region_[cur], segs_[sidx], (uint32_t)nseg, W, g, pr,
-                                   kFineBits, hist_, nullptr, nullptr, 0, cr, stream_));
+                                   kFineBits, hist_, nullptr, nullptr, 0, cr, table_slots, stream_));
         LAUNCH("scan", 12.0 * DF * W,
                launch_scan(hist_, DF, W, 1, offsets_, segs_[nsidx], partial_, cn, stream_));
         HIP_TRY(hipMemcpyAsync(h_counters_, cn, sizeof(uint64_t) * kNumCounters, hipMemcpyDeviceToHost,
@@ -420,7 +423,7 @@ int Engine::run(const RunSpec& spec, RunOut* out, std::string* err) {
           if (rc) return rc;
           LAUNCH("merge_scatter", R * (C + C_next),
                  launch_merge_expand(kMergeScatter, L, region_[cur], segs_[sidx], (uint32_t)nseg, W, g, pr, d1,
-                                     nullptr, offsets_, region_[to], 0, cr, stream_));
+                                     nullptr, offsets_, region_[to], 0, cr, table_slots, stream_));
           sidx = nsidx;
           rc = partition_to_final(to, d1, B, C_next);
           if (rc) return rc;
@@ -428,7 +431,7 @@ int Engine::run(const RunSpec& spec, RunOut* out, std::string* err) {
           // No successor survives: merge for the statistics only.
           LAUNCH("merge_last", R * C,
                  launch_merge_expand(kMergeLast, L, region_[cur], segs_[sidx], (uint32_t)nseg, W, g, pr, 0,
-                                     nullptr, nullptr, nullptr, 0, cr, stream_));
+                                     nullptr, nullptr, nullptr, 0, cr, table_slots, stream_));
         }
       }
       HIP_TRY(hipMemcpyAsync(h_counters_, cr, sizeof(uint64_t) * kNumCounters, hipMemcpyDeviceToHost, stream_));

@@ -24,8 +24,6 @@ constexpr int kMaxLanes = 4;
 constexpr uint64_t kEmptyKey = ~0ull;
 constexpr uint32_t kFineBits = 10;       // histogram resolution of the COUNT pass
 constexpr uint32_t kMaxDigitBits = 11;   // widest partition digit
-constexpr uint32_t kMergeSlots = 4096;   // LDS table slots per merge bucket
-constexpr uint32_t kMergeCap = 2560;     // records per merge pass (5/8 load)
 
 // Geometry of one row step (uniform across the launch).
 struct RowGeom {
@@ -85,10 +83,13 @@ hipError_t launch_partition(int lanes, const void* in, void* out, const uint64_t
 // COUNT: hist[(1<<kFineBits)][W] of successor digits.  SCATTER: successors
 // written to out at offsets[(1<<digit_bits)][W].  LAST: no expansion;
 // final_mask selects the states summed into kFinal0.
+// table_slots selects the LDS table (2048 or 4096 slots); a bucket of up to
+// merge_cap(table_slots) records is merged in one pass.
 hipError_t launch_merge_expand(int mode, int lanes, const void* in, const uint64_t* segs,
                                uint32_t nseg, uint32_t W, const RowGeom& g, const Primes& pr,
                                uint32_t digit_bits, uint32_t* hist, const uint64_t* offsets,
                                void* out, uint64_t final_mask, uint64_t* counters,
-                               hipStream_t stream);
+                               uint32_t table_slots, hipStream_t stream);
+uint32_t merge_cap(uint32_t table_slots);
 
 }  // namespace peps

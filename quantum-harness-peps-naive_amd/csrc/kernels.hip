@@ -464,43 +464,49 @@ __global__ __launch_bounds__(kBlock) void k_partition(const Rec<L>* __restrict__
 // ----------------------------------------------------------- merge-expand --
 // Workgroup w owns the buckets [w*per, (w+1)*per) of row r; the COUNT and
 // SCATTER launches use the same assignment, so they see identical merged
-// states.  Each bucket's records are inserted into an LDS open-addressing
-// table (linear probing on the mix's low bits); residues of equal keys are
-// added modulo each prime.  A bucket above the table's safe load is merged in
-// passes that each take the keys whose mix falls in one residue class.  The
-// merged states are then counted (layer statistics) and expanded through the
-// next row; the successors' digits index the next row's level-1 buckets.
+// states.  Per bucket:
+//   1. every record inserts its key into an LDS open-addressing table (linear
+//      probing on the mix's low bits); the record that claims an empty slot
+//      is the slot's owner and appends the slot to a list of unique slots
+//      (one LDS atomic per wave);
+//   2. owners store their residues into the slot (plain stores);
+//   3. the rare duplicates add theirs modulo each prime;
+//   4. the unique list is walked: layer statistics, then every merged state
+//      is expanded through the next row and its successors are counted
+//      (COUNT) or written (SCATTER) by next-row level-1 digit.
+// A bucket above the table's safe load is merged in passes that each take
+// the keys whose mix falls in one class (bits 12-35), so passes see disjoint
+// keys.  Probe exhaustion sets kError; the call then fails.
 
-constexpr int kMergeBatch = kMergeCap / kBlock;  // records per thread per load batch
-
-template <int L, int MODE>
+template <int L, int MODE, int T>
 __global__ __launch_bounds__(kBlock) void k_merge_expand(
     const Rec<L>* __restrict__ in, const uint64_t* __restrict__ segs, uint32_t nseg, uint32_t per,
     RowGeom g, Primes pr, uint32_t digit_bits, uint32_t* __restrict__ hist,
     const uint64_t* __restrict__ offsets, Rec<L>* __restrict__ out, uint64_t final_mask,
     uint64_t* __restrict__ counters) {
-  constexpr uint32_t T = kMergeSlots;
-  constexpr int LL = MODE == kMergeCount ? 1 : L;  // COUNT needs no residues
+  constexpr uint32_t kCap = (T * 5) / 8;             // records per pass
+  constexpr int KB = (int)((kCap + kBlock - 1) / kBlock);  // records per thread per batch
+  constexpr int LL = MODE == kMergeCount ? 1 : L;   // COUNT needs no residues
   constexpr uint32_t DM = MODE == kMergeLast ? 1 : (1u << kFineBits);
   __shared__ uint64_t tkey[T];
   __shared__ uint32_t tres[LL][T];
+  __shared__ uint16_t ulist[T];
   __shared__ uint32_t lcount[DM];
   __shared__ uint64_t gpos[MODE == kMergeScatter ? DM : 1];
   __shared__ uint64_t scratch[kWaves];
+  __shared__ uint32_t nuniq[2];
   const uint32_t D = MODE == kMergeLast ? 1u : (1u << digit_bits);
   const uint32_t W = gridDim.x;
+  const int lane = threadIdx.x & 63;
 
-  for (uint32_t i = threadIdx.x; i < T; i += kBlock) {
-    tkey[i] = kEmptyKey;
-#pragma unroll
-    for (int l = 0; l < LL; ++l) tres[l][i] = 0;
-  }
+  for (uint32_t i = threadIdx.x; i < T; i += kBlock) tkey[i] = kEmptyKey;
   if (MODE != kMergeLast) {
     for (uint32_t d = threadIdx.x; d < D; d += kBlock) {
       lcount[d] = 0;
       if (MODE == kMergeScatter) gpos[d] = offsets[(uint64_t)d * W + blockIdx.x];
     }
   }
+  if (threadIdx.x < 2) nuniq[threadIdx.x] = 0;
   lds_barrier();
 
   uint64_t uniq = 0;
@@ -508,15 +514,15 @@ __global__ __launch_bounds__(kBlock) void k_merge_expand(
 #pragma unroll
   for (int l = 0; l < L; ++l) wsum[l] = fsum[l] = 0;
   uint32_t overflow = 0;
+  uint32_t parity = 0;
 
   const uint32_t b0 = blockIdx.x * per;
   const uint32_t b1 = (b0 + per < nseg) ? b0 + per : nseg;
 
-  // Prefetch of the first bucket (when it fits one batch).
-  Rec<L> buf[kMergeBatch];
+  Rec<L> buf[KB];
   auto load_batch = [&](uint64_t base, uint64_t hi) {
 #pragma unroll
-    for (int k = 0; k < kMergeBatch; ++k) {
+    for (int k = 0; k < KB; ++k) {
       const uint64_t i = base + (uint64_t)k * kBlock + threadIdx.x;
       if (i < hi) {
         if (MODE == kMergeCount)
@@ -530,25 +536,28 @@ __global__ __launch_bounds__(kBlock) void k_merge_expand(
   if (b0 < b1) {
     plo = segs[b0];
     phi = segs[b0 + 1];
-    load_batch(plo, phi);
+    if (phi - plo <= kCap) load_batch(plo, phi);
   }
 
   for (uint32_t b = b0; b < b1; ++b) {
     const uint64_t lo = plo, hi = phi;
     const uint64_t m = hi - lo;
-    const uint32_t passes = m <= kMergeCap ? 1u : (uint32_t)((m + kMergeCap - 1) / kMergeCap);
+    const uint32_t passes = m <= kCap ? 1u : (uint32_t)((m + kCap - 1) / kCap);
     uint64_t nlo = 0, nhi = 0;
     if (b + 1 < b1) {
       nlo = segs[b + 1];
       nhi = segs[b + 2];
     }
     for (uint32_t pass = 0; pass < passes; ++pass) {
-      // ---- insert
-      for (uint64_t base = lo; base < hi; base += kMergeCap) {
-        const bool prefetched = (passes == 1);  // single-batch bucket already in buf
-        if (!prefetched) load_batch(base, hi);
+      uint32_t* const nu_ctr = &nuniq[parity];
+      for (uint64_t base = lo; base < hi; base += kCap) {
+        if (passes > 1) load_batch(base, hi);
+        // ---- 1. insert keys; owners append their slot to the unique list
+        uint32_t own = 0, dup = 0;
+        uint32_t sl[KB];
 #pragma unroll
-        for (int k = 0; k < kMergeBatch; ++k) {
+        for (int k = 0; k < KB; ++k) {
+          sl[k] = 0;
           const uint64_t i = base + (uint64_t)k * kBlock + threadIdx.x;
           if (i >= hi) continue;
           const uint64_t key = buf[k].key;
@@ -560,44 +569,79 @@ __global__ __launch_bounds__(kBlock) void k_merge_expand(
             const unsigned long long old = atomicCAS((unsigned long long*)&tkey[slot],
                                                      (unsigned long long)kEmptyKey,
                                                      (unsigned long long)key);
-            if (old == kEmptyKey || old == key) {
+            if (old == kEmptyKey) {
+              own |= 1u << k;
+              placed = true;
+              break;
+            }
+            if (old == key) {
+              dup |= 1u << k;
               placed = true;
               break;
             }
             slot = (slot + 1) & (T - 1);
           }
-          if (!placed) {
-            overflow = 1;
-            continue;
-          }
-          if (MODE != kMergeCount) {
+          if (!placed) overflow = 1;
+          sl[k] = slot;
+        }
 #pragma unroll
-            for (int l = 0; l < L; ++l) {
-              uint32_t* a = &tres[l < LL ? l : 0][slot];
-              uint32_t old = *a, assumed;
-              do {
-                assumed = old;
-                old = atomicCAS(a, assumed, addmod(assumed, buf[k].r[l], pr.p[l]));
-              } while (old != assumed);
-            }
+        for (int k = 0; k < KB; ++k) {
+          const bool o = (own >> k) & 1u;
+          const uint64_t bal = __ballot(o);
+          if (bal == 0) continue;
+          const int leader = __ffsll((long long)bal) - 1;
+          uint32_t base_idx = 0;
+          if (lane == leader) base_idx = atomicAdd(nu_ctr, (uint32_t)__popcll(bal));
+          base_idx = __shfl(base_idx, leader, 64);
+          if (o) {
+            const uint32_t r = __builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32),
+                                                         __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
+            ulist[base_idx + r] = (uint16_t)sl[k];
           }
         }
+        if (MODE != kMergeCount) {
+          lds_barrier();
+          // ---- 2. owners store their residues
+#pragma unroll
+          for (int k = 0; k < KB; ++k)
+            if ((own >> k) & 1u) {
+#pragma unroll
+              for (int l = 0; l < L; ++l) tres[l < LL ? l : 0][sl[k]] = buf[k].r[l];
+            }
+          lds_barrier();
+          // ---- 3. duplicates add modulo each prime
+          if (dup) {
+#pragma unroll
+            for (int k = 0; k < KB; ++k)
+              if ((dup >> k) & 1u) {
+#pragma unroll
+                for (int l = 0; l < L; ++l) {
+                  uint32_t* a = &tres[l < LL ? l : 0][sl[k]];
+                  uint32_t old = *a, assumed;
+                  do {
+                    assumed = old;
+                    old = atomicCAS(a, assumed, addmod(assumed, buf[k].r[l], pr.p[l]));
+                  } while (old != assumed);
+                }
+              }
+          }
+        }
+        lds_barrier();
       }
-      lds_barrier();
-      // Prefetch the next bucket while this one is expanded.
-      if (pass + 1 == passes && b + 1 < b1 && nhi - nlo <= kMergeCap) load_batch(nlo, nhi);
-      // ---- statistics, expansion, reset
-      for (uint32_t s = threadIdx.x; s < T; s += kBlock) {
-        const uint64_t key = tkey[s];
-        if (key == kEmptyKey) continue;
+      // The batch registers are free: prefetch the next bucket.
+      if (pass + 1 == passes && b + 1 < b1 && nhi - nlo <= kCap) load_batch(nlo, nhi);
+      // ---- 4. statistics and expansion of the merged states
+      const uint32_t nu = *nu_ctr;
+      for (uint32_t u = threadIdx.x; u < nu; u += kBlock) {
+        const uint32_t slot = ulist[u];
+        const uint64_t key = tkey[slot];
         uniq += 1;
         uint32_t v[L];
         if (MODE != kMergeCount) {
 #pragma unroll
           for (int l = 0; l < L; ++l) {
-            v[l] = tres[l < LL ? l : 0][s];
+            v[l] = tres[l < LL ? l : 0][slot];
             wsum[l] += v[l];
-            tres[l < LL ? l : 0][s] = 0;
           }
           if (MODE == kMergeLast && (key & final_mask) == final_mask) {
 #pragma unroll
@@ -611,8 +655,10 @@ __global__ __launch_bounds__(kBlock) void k_merge_expand(
             free ^= bit;
             const uint64_t sk = successor(key, bit, g);
             const uint32_t d = top_digit(sk, digit_bits);
-            const uint32_t rank = atomicAdd(&lcount[d], 1u);
-            if (MODE == kMergeScatter) {
+            if (MODE == kMergeCount) {
+              atomicAdd(&lcount[d], 1u);
+            } else {
+              const uint32_t rank = atomicAdd(&lcount[d], 1u);
               Rec<L> rec;
               rec.key = sk;
 #pragma unroll
@@ -621,12 +667,11 @@ __global__ __launch_bounds__(kBlock) void k_merge_expand(
             }
           }
         }
-        tkey[s] = kEmptyKey;
+        tkey[slot] = kEmptyKey;
       }
+      if (threadIdx.x == 0) nuniq[parity ^ 1] = 0;
+      parity ^= 1;
       lds_barrier();
-    }
-    if (b + 1 < b1 && nhi - nlo > kMergeCap) {
-      // multi-pass bucket: loaded batch by batch inside its passes
     }
     plo = nlo;
     phi = nhi;
@@ -729,7 +774,7 @@ hipError_t launch_partition(int lanes, const void* in, void* out, const uint64_t
   return hipGetLastError();
 }
 
-template <int L>
+template <int L, int T>
 static hipError_t launch_me(int mode, dim3 grid, hipStream_t stream, const void* in,
                             const uint64_t* segs, uint32_t nseg, uint32_t per, const RowGeom& g,
                             const Primes& pr, uint32_t digit_bits, uint32_t* hist,
@@ -738,29 +783,45 @@ static hipError_t launch_me(int mode, dim3 grid, hipStream_t stream, const void*
   const Rec<L>* i = (const Rec<L>*)in;
   Rec<L>* o = (Rec<L>*)out;
   switch (mode) {
-    case kMergeCount: hipLaunchKernelGGL((k_merge_expand<L, kMergeCount>), grid, dim3(kBlock), 0, stream, i, segs, nseg, per, g, pr, digit_bits, hist, offsets, o, final_mask, counters); break;
-    case kMergeScatter: hipLaunchKernelGGL((k_merge_expand<L, kMergeScatter>), grid, dim3(kBlock), 0, stream, i, segs, nseg, per, g, pr, digit_bits, hist, offsets, o, final_mask, counters); break;
-    case kMergeLast: hipLaunchKernelGGL((k_merge_expand<L, kMergeLast>), grid, dim3(kBlock), 0, stream, i, segs, nseg, per, g, pr, digit_bits, hist, offsets, o, final_mask, counters); break;
+    case kMergeCount: hipLaunchKernelGGL((k_merge_expand<L, kMergeCount, T>), grid, dim3(kBlock), 0, stream, i, segs, nseg, per, g, pr, digit_bits, hist, offsets, o, final_mask, counters); break;
+    case kMergeScatter: hipLaunchKernelGGL((k_merge_expand<L, kMergeScatter, T>), grid, dim3(kBlock), 0, stream, i, segs, nseg, per, g, pr, digit_bits, hist, offsets, o, final_mask, counters); break;
+    case kMergeLast: hipLaunchKernelGGL((k_merge_expand<L, kMergeLast, T>), grid, dim3(kBlock), 0, stream, i, segs, nseg, per, g, pr, digit_bits, hist, offsets, o, final_mask, counters); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
 }
 
+uint32_t merge_cap(uint32_t table_slots) { return (table_slots * 5) / 8; }
+
 hipError_t launch_merge_expand(int mode, int lanes, const void* in, const uint64_t* segs,
                                uint32_t nseg, uint32_t W, const RowGeom& g, const Primes& pr,
                                uint32_t digit_bits, uint32_t* hist, const uint64_t* offsets,
                                void* out, uint64_t final_mask, uint64_t* counters,
-                               hipStream_t stream) {
+                               uint32_t table_slots, hipStream_t stream) {
   if (nseg == 0 || W == 0 || digit_bits > kFineBits) return hipErrorInvalidValue;
   const uint32_t per = (nseg + W - 1) / W;
   dim3 grid(W);
-  switch (lanes) {
-    case 1: return launch_me<1>(mode, grid, stream, in, segs, nseg, per, g, pr, digit_bits, hist, offsets, out, final_mask, counters);
-    case 2: return launch_me<2>(mode, grid, stream, in, segs, nseg, per, g, pr, digit_bits, hist, offsets, out, final_mask, counters);
-    case 3: return launch_me<3>(mode, grid, stream, in, segs, nseg, per, g, pr, digit_bits, hist, offsets, out, final_mask, counters);
-    case 4: return launch_me<4>(mode, grid, stream, in, segs, nseg, per, g, pr, digit_bits, hist, offsets, out, final_mask, counters);
-    default: return hipErrorInvalidValue;
+#define PEPS_ME(LN, TS) return launch_me<LN, TS>(mode, grid, stream, in, segs, nseg, per, g, pr, digit_bits, hist, offsets, out, final_mask, counters)
+  if (table_slots == 2048) {
+    switch (lanes) {
+      case 1: PEPS_ME(1, 2048);
+      case 2: PEPS_ME(2, 2048);
+      case 3: PEPS_ME(3, 2048);
+      case 4: PEPS_ME(4, 2048);
+      default: return hipErrorInvalidValue;
+    }
   }
+  if (table_slots == 4096) {
+    switch (lanes) {
+      case 1: PEPS_ME(1, 4096);
+      case 2: PEPS_ME(2, 4096);
+      case 3: PEPS_ME(3, 4096);
+      case 4: PEPS_ME(4, 4096);
+      default: return hipErrorInvalidValue;
+    }
+  }
+#undef PEPS_ME
+  return hipErrorInvalidValue;
 }
 
 }  // namespace peps
