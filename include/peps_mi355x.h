@@ -108,8 +108,8 @@ enum { PEPS_SYMMETRY_NONE = 0, PEPS_SYMMETRY_TOP_ROW_ORBITS = 1 };
 /* Tuning and resource options.  Zero-initialised means "defaults". */
 typedef struct {
   uint32_t crt_lanes;        /* 0: reference plan (shortest prime prefix with M > N!); else 1..4 */
-  uint32_t radix_bits;       /* hash-radix digit width per partition level (0: auto; 4..11) */
-  uint32_t tile_states;      /* target records per LDS merge bucket (0: auto; 256..4096) */
+  uint32_t radix_bits;       /* hash-radix digit width per partition level (0: auto; 1..11) */
+  uint32_t tile_states;      /* target records per LDS merge bucket (0: auto; 256..2560) */
   uint32_t device;           /* HIP device ordinal */
   uint64_t hbm_budget_bytes; /* arena cap (0: auto, 85% of free HBM) */
   uint32_t collect_layers;   /* nonzero: fill per-layer metrics (needs a host sync per row) */

@@ -64,8 +64,8 @@ int resolve(unsigned n, const peps_options* o, Resolved* r, std::string* err) {
       r->primes.assign(peps::crt_primes(), peps::crt_primes() + o->crt_lanes);
     }
     if (o->radix_bits) {
-      if (o->radix_bits < 1 || o->radix_bits > 10) {
-        *err = "radix_bits must be in 1..10";
+      if (o->radix_bits < 1 || o->radix_bits > 11) {
+        *err = "radix_bits must be in 1..11";
         return PEPS_E_INVALID;
       }
       r->radix_bits = o->radix_bits;
