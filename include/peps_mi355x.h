@@ -113,7 +113,8 @@ typedef struct {
   uint32_t device;           /* HIP device ordinal */
   uint64_t hbm_budget_bytes; /* arena cap (0: auto, 85% of free HBM) */
   uint32_t collect_layers;   /* nonzero: fill per-layer metrics (needs a host sync per row) */
-  uint32_t reserved[7];
+  uint32_t expand_bits;      /* max digit width the merge-expand scatter writes (0: auto; 1..10) */
+  uint32_t reserved[6];
 } peps_options;
 
 /* ---------------------------------------------------------------- solve -- */

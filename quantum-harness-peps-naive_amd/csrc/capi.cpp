@@ -42,6 +42,7 @@ struct Resolved {
   std::vector<uint64_t> primes;
   uint32_t radix_bits = 8;
   uint32_t tile_states = 1024;
+  uint32_t expand_bits = 10;
   uint32_t device = 0;
   uint64_t budget = 0;
 };
@@ -76,6 +77,13 @@ int resolve(unsigned n, const peps_options* o, Resolved* r, std::string* err) {
         return PEPS_E_INVALID;
       }
       r->tile_states = o->tile_states;
+    }
+    if (o->expand_bits) {
+      if (o->expand_bits > 10) {
+        *err = "expand_bits must be in 1..10";
+        return PEPS_E_INVALID;
+      }
+      r->expand_bits = o->expand_bits;
     }
     r->device = o->device;
     r->budget = o->hbm_budget_bytes;
@@ -140,6 +148,7 @@ int run_engine(const Resolved& r, peps::RunSpec& spec, peps::RunOut* ro, std::st
   spec.primes = r.primes;
   spec.radix_bits = r.radix_bits;
   spec.tile_states = r.tile_states;
+  spec.expand_bits = r.expand_bits;
   spec.budget_bytes = r.budget;
   std::lock_guard<std::mutex> lk(eng->mutex());
   return eng->run(spec, ro, err);

@@ -33,7 +33,8 @@ struct RunSpec {
   unsigned slice_bits = 0;         // slice ids occupy bits [3n, 3n + slice_bits)
   std::vector<uint64_t> primes;    // CRT lanes (1..4)
   uint32_t tile_states = 2048;     // target records per LDS merge bucket
-  uint32_t radix_bits = 8;         // max hash-radix digit width per level
+  uint32_t radix_bits = 8;         // max hash-radix digit width per partition level
+  uint32_t expand_bits = 10;       // max digit width written by the merge-expand scatter
   uint64_t budget_bytes = 0;       // arena cap (0: 85% of free HBM at first use)
 };
 

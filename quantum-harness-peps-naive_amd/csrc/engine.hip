@@ -412,7 +412,7 @@ int Engine::run(const RunSpec& spec, RunOut* out, std::string* err) {
         C_next = h_counters_[kCandidates];
         if (C_next) {
           const unsigned B = total_bits(C_next);
-          const unsigned d1 = std::min<unsigned>(B, kFineBits);
+          const unsigned d1 = std::min<unsigned>(B, std::min<unsigned>(kFineBits, std::max(1u, spec.expand_bits)));
           if (d1 < kFineBits) {
             LAUNCH("scan", 12.0 * DF * W,
                    launch_scan(hist_, 1u << d1, W, 1u << (kFineBits - d1), offsets_, segs_[nsidx], partial_,

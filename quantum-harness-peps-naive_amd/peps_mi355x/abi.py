@@ -107,7 +107,8 @@ class Options(ctypes.Structure):
         ("device", ctypes.c_uint32),
         ("hbm_budget_bytes", ctypes.c_uint64),
         ("collect_layers", ctypes.c_uint32),
-        ("reserved", ctypes.c_uint32 * 7),
+        ("expand_bits", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32 * 6),
     ]
 
 

@@ -43,6 +43,7 @@ class PepsOptions:
     tile_states: int = 0        # 0: default LDS merge bucket size
     device: int = 0
     hbm_budget_bytes: int = 0
+    expand_bits: int = 0        # 0: default merge-expand scatter fan-out
 
     def to_c(self) -> Options:
         o = Options()
@@ -52,6 +53,7 @@ class PepsOptions:
         o.device = self.device
         o.hbm_budget_bytes = self.hbm_budget_bytes
         o.collect_layers = 1
+        o.expand_bits = self.expand_bits
         return o
 
 
