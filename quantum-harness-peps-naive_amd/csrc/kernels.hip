@@ -495,6 +495,7 @@ __global__ __launch_bounds__(kBlock) void k_merge_expand(
   __shared__ uint64_t gpos[MODE == kMergeScatter ? DM : 1];
   __shared__ uint64_t scratch[kWaves];
   __shared__ uint32_t nuniq[2];
+  __shared__ uint16_t wq[kWaves][64];  // per-wave (lane << 5 | column) expansion items
   const uint32_t D = MODE == kMergeLast ? 1u : (1u << digit_bits);
   const uint32_t W = gridDim.x;
   const int lane = threadIdx.x & 63;
@@ -630,44 +631,84 @@ __global__ __launch_bounds__(kBlock) void k_merge_expand(
       }
       // The batch registers are free: prefetch the next bucket.
       if (pass + 1 == passes && b + 1 < b1 && nhi - nlo <= kCap) load_batch(nlo, nhi);
-      // ---- 4. statistics and expansion of the merged states
+      // ---- 4. statistics and expansion of the merged states.  The loop
+      // runs wave-uniformly; each wave turns its 64 states' legal columns
+      // into a compact (lane, column) work list so every lane expands one
+      // successor per step instead of looping over its own state's columns.
       const uint32_t nu = *nu_ctr;
-      for (uint32_t u = threadIdx.x; u < nu; u += kBlock) {
-        const uint32_t slot = ulist[u];
-        const uint64_t key = tkey[slot];
-        uniq += 1;
+      for (uint32_t ub = 0; ub < nu; ub += kBlock) {
+        const uint32_t u = ub + threadIdx.x;
+        const bool valid = u < nu;
+        uint64_t key = 0, free = 0;
         uint32_t v[L];
-        if (MODE != kMergeCount) {
 #pragma unroll
-          for (int l = 0; l < L; ++l) {
-            v[l] = tres[l < LL ? l : 0][slot];
-            wsum[l] += v[l];
-          }
-          if (MODE == kMergeLast && (key & final_mask) == final_mask) {
+        for (int l = 0; l < L; ++l) v[l] = 0;
+        if (valid) {
+          const uint32_t slot = ulist[u];
+          key = tkey[slot];
+          tkey[slot] = kEmptyKey;
+          uniq += 1;
+          if (MODE != kMergeCount) {
 #pragma unroll
-            for (int l = 0; l < L; ++l) fsum[l] += v[l];
-          }
-        }
-        if (MODE != kMergeLast) {
-          uint64_t free = legal_positions(key, g);
-          while (free) {
-            const uint64_t bit = free & (~free + 1);
-            free ^= bit;
-            const uint64_t sk = successor(key, bit, g);
-            const uint32_t d = top_digit(sk, digit_bits);
-            if (MODE == kMergeCount) {
-              atomicAdd(&lcount[d], 1u);
-            } else {
-              const uint32_t rank = atomicAdd(&lcount[d], 1u);
-              Rec<L> rec;
-              rec.key = sk;
+            for (int l = 0; l < L; ++l) {
+              v[l] = tres[l < LL ? l : 0][slot];
+              wsum[l] += v[l];
+            }
+            if (MODE == kMergeLast && (key & final_mask) == final_mask) {
 #pragma unroll
-              for (int l = 0; l < L; ++l) rec.r[l] = v[l];
-              out[gpos[d] + rank] = rec;
+              for (int l = 0; l < L; ++l) fsum[l] += v[l];
             }
           }
+          if (MODE != kMergeLast) free = legal_positions(key, g);
         }
-        tkey[slot] = kEmptyKey;
+        if (MODE != kMergeLast) {
+          const uint32_t cnt = (uint32_t)__popcll(free);
+          uint32_t pre = cnt;
+#pragma unroll
+          for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t y = __shfl_up(pre, o, 64);
+            if (lane >= o) pre += y;
+          }
+          const uint32_t tot = __shfl(pre, 63, 64);
+          pre -= cnt;
+          uint16_t* q = wq[threadIdx.x >> 6];
+          for (uint32_t w0 = 0; w0 < tot; w0 += 64) {
+            // this lane's items with wave index in [w0, w0 + 64)
+            if (cnt && pre < w0 + 64 && pre + cnt > w0) {
+              uint64_t f = free;
+              for (uint32_t j = 0; j < cnt; ++j) {
+                const uint32_t col = (uint32_t)__builtin_ctzll(f);
+                f &= f - 1;
+                const uint32_t idx = pre + j;
+                if (idx >= w0 && idx < w0 + 64) q[idx - w0] = (uint16_t)((lane << 5) | col);
+              }
+            }
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+            const uint32_t it = w0 + lane;
+            const uint32_t item = it < tot ? q[lane] : 0u;
+            const int src = (int)(item >> 5);
+            const uint64_t skey = __shfl(key, src, 64);
+            uint32_t sv[L];
+#pragma unroll
+            for (int l = 0; l < L; ++l) sv[l] = MODE == kMergeScatter ? __shfl(v[l], src, 64) : 0u;
+            if (it < tot) {
+              const uint64_t sk = successor(skey, (uint64_t)1 << (item & 31u), g);
+              const uint32_t d = top_digit(sk, digit_bits);
+              if (MODE == kMergeCount) {
+                atomicAdd(&lcount[d], 1u);
+              } else {
+                const uint32_t rank = atomicAdd(&lcount[d], 1u);
+                Rec<L> rec;
+                rec.key = sk;
+#pragma unroll
+                for (int l = 0; l < L; ++l) rec.r[l] = sv[l];
+                out[gpos[d] + rank] = rec;
+              }
+            }
+            __builtin_amdgcn_wave_barrier();
+          }
+        }
       }
       if (threadIdx.x == 0) nuniq[parity ^ 1] = 0;
       parity ^= 1;
