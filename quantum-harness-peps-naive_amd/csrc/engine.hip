@@ -239,15 +239,15 @@ int Engine::run(const RunSpec& spec, RunOut* out, std::string* err) {
 
   RowGeom g{};
   g.n = n;
-  g.board_mask = mask_bits(n);
+  g.board_mask = (uint32_t)mask_bits(n);
   g.state_mask = mask_bits(3 * n);
   g.orbit_mask = g.board_mask;
-  g.col_in = occupied_.column_in;
-  g.col_out = occupied_.column_out;
-  g.dr_in = occupied_.diag_dr_in;
-  g.dr_out = occupied_.diag_dr_out;
-  g.dl_in = occupied_.diag_dl_in;
-  g.dl_out = occupied_.diag_dl_out;
+  g.col_in = occupied_.column_in ? g.board_mask : 0u;
+  g.col_out = occupied_.column_out ? g.board_mask : 0u;
+  g.dr_in = occupied_.diag_dr_in ? g.board_mask : 0u;
+  g.dr_out = occupied_.diag_dr_out ? g.board_mask : 0u;
+  g.dl_in = occupied_.diag_dl_in ? g.board_mask : 0u;
+  g.dl_out = occupied_.diag_dl_out ? g.board_mask : 0u;
 
   const uint32_t rbits = std::max<uint32_t>(1, std::min<uint32_t>(spec.radix_bits, kMaxDigitBits));
   // LDS merge table: 2048 slots (more workgroups per CU) for small tiles,
@@ -295,8 +295,8 @@ int Engine::run(const RunSpec& spec, RunOut* out, std::string* err) {
   if (orbit) {
     LAUNCH("count", 8.0 * S0, launch_count(skeys, S0, g, c0 + kMatchedSlot, stream_));
     g.orbit_row = 1;
-    g.orbit_mask = mask_bits((n - 1) / 2 + 1);  // columns c <= n-1-c
-    g.centre_bit = (n % 2) ? ((uint64_t)1 << (n / 2)) : 0;
+    g.orbit_mask = (uint32_t)mask_bits((n - 1) / 2 + 1);  // columns c <= n-1-c
+    g.centre_bit = (n % 2) ? (1u << (n / 2)) : 0u;
   }
   LAUNCH("count", 8.0 * S0, launch_count(skeys, S0, g, c0, stream_));
   HIP_TRY(hipMemcpyAsync(h_counters_, c0, sizeof(uint64_t) * kNumCounters, hipMemcpyDeviceToHost, stream_));
@@ -387,7 +387,7 @@ int Engine::run(const RunSpec& spec, RunOut* out, std::string* err) {
       if (last) {
         LAUNCH("merge_last", R * C,
                launch_merge_expand(kMergeLast, L, region_[cur], segs_[sidx], (uint32_t)nseg, W, g, pr, 0,
-                                   nullptr, nullptr, nullptr, g.board_mask, cr, table_slots, stream_));
+                                   nullptr, nullptr, nullptr, (uint64_t)g.board_mask, cr, table_slots, stream_));
       } else {
         // COUNT: successor digits of the merged states at the fine resolution.
         const uint32_t DF = 1u << kFineBits;

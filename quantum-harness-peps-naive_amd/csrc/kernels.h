@@ -25,16 +25,18 @@ constexpr uint64_t kEmptyKey = ~0ull;
 constexpr uint32_t kFineBits = 10;       // histogram resolution of the COUNT pass
 constexpr uint32_t kMaxDigitBits = 11;   // widest partition digit
 
-// Geometry of one row step (uniform across the launch).
+// Geometry of one row step (uniform across the launch).  Board masks are
+// 32-bit (N <= 21).  The occupied C entry's binary legs (src/lib.rs:425-485)
+// are stored as masks: x_in = board mask if the incoming leg is 1 else 0, so
+// "bit matches the leg" is ~(bit ^ x_in); x_out likewise for the outgoing leg.
 struct RowGeom {
   uint32_t n;
   uint32_t orbit_row;    // 1: apply the row-0 vertical-reflection orbit weights
-  uint64_t board_mask;   // (1 << n) - 1
+  uint32_t board_mask;   // (1 << n) - 1
+  uint32_t orbit_mask;   // legal-position restriction (left half on the orbit row)
+  uint32_t centre_bit;   // the odd-N fixed column on the orbit row (weight 1), else 0
+  uint32_t col_in, col_out, dr_in, dr_out, dl_in, dl_out;  // leg masks
   uint64_t state_mask;   // (1 << 3n) - 1; bits above are a slice id
-  uint64_t orbit_mask;   // legal-position restriction (left half on the orbit row)
-  uint64_t centre_bit;   // the odd-N fixed column on the orbit row (weight 1), else 0
-  // Occupied C entry of the compiled operator (binary legs, src/lib.rs:425-485).
-  uint32_t col_in, col_out, dr_in, dr_out, dl_in, dl_out;
 };
 
 struct Primes {

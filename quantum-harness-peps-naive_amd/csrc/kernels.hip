@@ -44,14 +44,14 @@ __device__ __forceinline__ void lds_barrier() {
   asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
 }
 
-// MurmurHash3's 64-bit finalizer (public domain): a bijection on u64, so
-// partitioning by its bits never separates equal keys.
+// Bijective 64-bit mix (xorshift, odd multiply, xorshift): equal keys share
+// every derived digit and slot; distinct keys spread uniformly over the top
+// bits (radix digits), bits 12-35 (merge-pass class) and the low bits (LDS
+// table slot).
 __device__ __forceinline__ uint64_t mix64(uint64_t k) {
-  k ^= k >> 33;
-  k *= 0xff51afd7ed558ccdull;
-  k ^= k >> 33;
-  k *= 0xc4ceb9fe1a85ec53ull;
-  k ^= k >> 33;
+  k ^= k >> 31;
+  k *= 0x9e3779b97f4a7c15ull;
+  k ^= k >> 29;
   return k;
 }
 
@@ -60,27 +60,35 @@ __device__ __forceinline__ uint32_t addmod(uint32_t a, uint32_t b, uint32_t p) {
   return (uint32_t)(s >= p ? s - p : s);
 }
 
-__device__ __forceinline__ uint64_t match_bits(uint64_t mask, uint32_t required, uint64_t board) {
-  return required ? (mask & board) : (~mask & board);
+// A boundary key unpacked into its three N-bit signal masks.
+struct Board {
+  uint32_t cols, dr, dl;
+};
+
+__device__ __forceinline__ Board unpack(uint64_t key, const RowGeom& g) {
+  return Board{(uint32_t)key & g.board_mask, (uint32_t)(key >> g.n) & g.board_mask,
+               (uint32_t)(key >> (2 * g.n)) & g.board_mask};
 }
 
-// Legal positions of the occupied entry for a parent key (src/lib.rs:611-619,
-// the sparse form src/lib.rs:656-663).
-__device__ __forceinline__ uint64_t legal_positions(uint64_t key, const RowGeom& g) {
-  const uint64_t bm = g.board_mask;
-  const uint64_t cols = key & bm, dr = (key >> g.n) & bm, dl = (key >> (2 * g.n)) & bm;
-  return match_bits(cols, g.col_in, bm) & match_bits(dr, g.dr_in, bm) & match_bits(dl, g.dl_in, bm) &
-         g.orbit_mask;
+// Columns whose three incoming legs match the occupied entry's input legs
+// (src/lib.rs:611-619; the sparse form src/lib.rs:656-663).
+__device__ __forceinline__ uint32_t legal_positions(const Board& s, const RowGeom& g) {
+  return ~((s.cols ^ g.col_in) | (s.dr ^ g.dr_in) | (s.dl ^ g.dl_in)) & g.orbit_mask;
 }
 
-// Successor boundary for the single set bit `b` (src/lib.rs:622-635).
-__device__ __forceinline__ uint64_t successor(uint64_t key, uint64_t b, const RowGeom& g) {
-  const uint64_t bm = g.board_mask;
-  const uint64_t cols = key & bm, dr = (key >> g.n) & bm, dl = (key >> (2 * g.n)) & bm;
-  const uint64_t ncols = g.col_out ? (cols | b) : (cols & ~b);
-  const uint64_t ndr = ((g.dr_out ? (dr | b) : (dr & ~b)) << 1) & bm;
-  const uint64_t ndl = (g.dl_out ? (dl | b) : (dl & ~b)) >> 1;
-  return (key & ~g.state_mask) | ncols | (ndr << g.n) | (ndl << (2 * g.n));
+__device__ __forceinline__ uint32_t legal_positions(uint64_t key, const RowGeom& g) {
+  return legal_positions(unpack(key, g), g);
+}
+
+// Successor boundary for the single column bit `b` (src/lib.rs:622-635): the
+// output legs are written at b, diag_dr shifts right-down (<< 1, the bit
+// leaving the board is contracted with v2), diag_dl shifts left-down (>> 1).
+__device__ __forceinline__ uint64_t successor(uint64_t key, const Board& s, uint32_t b,
+                                              const RowGeom& g) {
+  const uint32_t c = (s.cols & ~b) | (b & g.col_out);
+  const uint32_t dr = (((s.dr & ~b) | (b & g.dr_out)) << 1) & g.board_mask;
+  const uint32_t dl = ((s.dl & ~b) | (b & g.dl_out)) >> 1;
+  return (key & ~g.state_mask) | (uint64_t)c | ((uint64_t)dr << g.n) | ((uint64_t)dl << (2 * g.n));
 }
 
 __device__ __forceinline__ uint32_t top_digit(uint64_t key, uint32_t bits) {
@@ -218,7 +226,7 @@ __global__ __launch_bounds__(kBlock) void k_count(const uint64_t* __restrict__ k
   uint64_t c = 0;
   for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < S;
        i += (uint64_t)gridDim.x * kBlock)
-    c += __popcll(legal_positions(keys[i], g));
+    c += __popc(legal_positions(keys[i], g));
   c = block_sum_u64(c, scratch);
   if (threadIdx.x == 0 && c) atomicAdd((unsigned long long*)&counters[kCandidates], (unsigned long long)c);
 }
@@ -239,11 +247,12 @@ __global__ __launch_bounds__(kBlock) void k_expand_hist(const uint64_t* __restri
   const uint64_t hi = lo + per_tile < S ? lo + per_tile : S;
   for (uint64_t i = lo + threadIdx.x; i < hi; i += kBlock) {
     const uint64_t key = keys[i];
-    uint64_t free = legal_positions(key, g);
+    const Board bs = unpack(key, g);
+    uint32_t free = legal_positions(bs, g);
     while (free) {
-      const uint64_t b = free & (~free + 1);
+      const uint32_t b = free & (~free + 1);
       free ^= b;
-      atomicAdd(&lh[top_digit(successor(key, b, g), digit_bits)], 1u);
+      atomicAdd(&lh[top_digit(successor(key, bs, b, g), digit_bits)], 1u);
     }
   }
   lds_barrier();
@@ -269,15 +278,18 @@ __global__ __launch_bounds__(kBlock) void k_expand_scatter(
   uint32_t staged = 0;
   for (uint64_t base = lo; base < hi; base += kBlock) {
     const uint64_t i = base + threadIdx.x;
-    uint64_t key = 0, free = 0;
+    uint64_t key = 0;
+    uint32_t free = 0;
+    Board bs{0, 0, 0};
     uint32_t r[L];
     if (i < hi) {
       key = keys[i];
-      free = legal_positions(key, g);
+      bs = unpack(key, g);
+      free = legal_positions(bs, g);
 #pragma unroll
       for (int l = 0; l < L; ++l) r[l] = res[(uint64_t)l * res_stride + i];
     }
-    const uint32_t cnt = (uint32_t)__popcll(free);
+    const uint32_t cnt = (uint32_t)__popc(free);
     uint32_t total;
     const uint32_t pos = block_exclusive_scan(cnt, &total, st.wave_tot);
     // Emit this round's successors through windows that fit the staging area.
@@ -290,14 +302,14 @@ __global__ __launch_bounds__(kBlock) void k_expand_scatter(
       const uint32_t space = kStage - staged;
       const uint32_t wend = (total - win < space) ? total : win + space;
       if (cnt && pos < wend && pos + cnt > win) {
-        uint64_t f = free;
+        uint32_t f = free;
         for (uint32_t j = 0; j < cnt; ++j) {
-          const uint64_t b = f & (~f + 1);
+          const uint32_t b = f & (~f + 1);
           f ^= b;
           const uint32_t idx = pos + j;
           if (idx < win || idx >= wend) continue;
           Rec<L> rec;
-          rec.key = successor(key, b, g);
+          rec.key = successor(key, bs, b, g);
           const bool twice = g.orbit_row && b != g.centre_bit;
 #pragma unroll
           for (int l = 0; l < L; ++l) rec.r[l] = twice ? addmod(r[l], r[l], pr.p[l]) : r[l];
@@ -495,7 +507,6 @@ __global__ __launch_bounds__(kBlock) void k_merge_expand(
   __shared__ uint64_t gpos[MODE == kMergeScatter ? DM : 1];
   __shared__ uint64_t scratch[kWaves];
   __shared__ uint32_t nuniq[2];
-  __shared__ uint16_t wq[kWaves][64];  // per-wave (lane << 5 | column) expansion items
   const uint32_t D = MODE == kMergeLast ? 1u : (1u << digit_bits);
   const uint32_t W = gridDim.x;
   const int lane = threadIdx.x & 63;
@@ -631,82 +642,43 @@ __global__ __launch_bounds__(kBlock) void k_merge_expand(
       }
       // The batch registers are free: prefetch the next bucket.
       if (pass + 1 == passes && b + 1 < b1 && nhi - nlo <= kCap) load_batch(nlo, nhi);
-      // ---- 4. statistics and expansion of the merged states.  The loop
-      // runs wave-uniformly; each wave turns its 64 states' legal columns
-      // into a compact (lane, column) work list so every lane expands one
-      // successor per step instead of looping over its own state's columns.
+      // ---- 4. statistics and expansion of the merged states
       const uint32_t nu = *nu_ctr;
-      for (uint32_t ub = 0; ub < nu; ub += kBlock) {
-        const uint32_t u = ub + threadIdx.x;
-        const bool valid = u < nu;
-        uint64_t key = 0, free = 0;
+      for (uint32_t u = threadIdx.x; u < nu; u += kBlock) {
+        const uint32_t slot = ulist[u];
+        const uint64_t key = tkey[slot];
+        tkey[slot] = kEmptyKey;
+        uniq += 1;
         uint32_t v[L];
+        if (MODE != kMergeCount) {
 #pragma unroll
-        for (int l = 0; l < L; ++l) v[l] = 0;
-        if (valid) {
-          const uint32_t slot = ulist[u];
-          key = tkey[slot];
-          tkey[slot] = kEmptyKey;
-          uniq += 1;
-          if (MODE != kMergeCount) {
-#pragma unroll
-            for (int l = 0; l < L; ++l) {
-              v[l] = tres[l < LL ? l : 0][slot];
-              wsum[l] += v[l];
-            }
-            if (MODE == kMergeLast && (key & final_mask) == final_mask) {
-#pragma unroll
-              for (int l = 0; l < L; ++l) fsum[l] += v[l];
-            }
+          for (int l = 0; l < L; ++l) {
+            v[l] = tres[l < LL ? l : 0][slot];
+            wsum[l] += v[l];
           }
-          if (MODE != kMergeLast) free = legal_positions(key, g);
+          if (MODE == kMergeLast && (key & final_mask) == final_mask) {
+#pragma unroll
+            for (int l = 0; l < L; ++l) fsum[l] += v[l];
+          }
         }
         if (MODE != kMergeLast) {
-          const uint32_t cnt = (uint32_t)__popcll(free);
-          uint32_t pre = cnt;
+          const Board bs = unpack(key, g);
+          uint32_t free = legal_positions(bs, g);
+          while (free) {
+            const uint32_t bit = free & (~free + 1);
+            free ^= bit;
+            const uint64_t sk = successor(key, bs, bit, g);
+            const uint32_t d = top_digit(sk, digit_bits);
+            if (MODE == kMergeCount) {
+              atomicAdd(&lcount[d], 1u);
+            } else {
+              const uint32_t rank = atomicAdd(&lcount[d], 1u);
+              Rec<L> rec;
+              rec.key = sk;
 #pragma unroll
-          for (int o = 1; o < 64; o <<= 1) {
-            const uint32_t y = __shfl_up(pre, o, 64);
-            if (lane >= o) pre += y;
-          }
-          const uint32_t tot = __shfl(pre, 63, 64);
-          pre -= cnt;
-          uint16_t* q = wq[threadIdx.x >> 6];
-          for (uint32_t w0 = 0; w0 < tot; w0 += 64) {
-            // this lane's items with wave index in [w0, w0 + 64)
-            if (cnt && pre < w0 + 64 && pre + cnt > w0) {
-              uint64_t f = free;
-              for (uint32_t j = 0; j < cnt; ++j) {
-                const uint32_t col = (uint32_t)__builtin_ctzll(f);
-                f &= f - 1;
-                const uint32_t idx = pre + j;
-                if (idx >= w0 && idx < w0 + 64) q[idx - w0] = (uint16_t)((lane << 5) | col);
-              }
+              for (int l = 0; l < L; ++l) rec.r[l] = v[l];
+              out[gpos[d] + rank] = rec;
             }
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-            const uint32_t it = w0 + lane;
-            const uint32_t item = it < tot ? q[lane] : 0u;
-            const int src = (int)(item >> 5);
-            const uint64_t skey = __shfl(key, src, 64);
-            uint32_t sv[L];
-#pragma unroll
-            for (int l = 0; l < L; ++l) sv[l] = MODE == kMergeScatter ? __shfl(v[l], src, 64) : 0u;
-            if (it < tot) {
-              const uint64_t sk = successor(skey, (uint64_t)1 << (item & 31u), g);
-              const uint32_t d = top_digit(sk, digit_bits);
-              if (MODE == kMergeCount) {
-                atomicAdd(&lcount[d], 1u);
-              } else {
-                const uint32_t rank = atomicAdd(&lcount[d], 1u);
-                Rec<L> rec;
-                rec.key = sk;
-#pragma unroll
-                for (int l = 0; l < L; ++l) rec.r[l] = sv[l];
-                out[gpos[d] + rank] = rec;
-              }
-            }
-            __builtin_amdgcn_wave_barrier();
           }
         }
       }
