@@ -23,6 +23,29 @@ sys.path.insert(0, os.path.join(ROOT, "quantum-harness-peps-naive_amd"))
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, vendor figure
 
+# rocprofv3 kernel names of the engine's profiling labels (L lanes, T slots).
+ROCPROF_NAMES = {
+    "merge_count": "k_merge_expand<{L}, 0, {T}>",
+    "merge_scatter": "k_merge_expand<{L}, 1, {T}>",
+    "merge_last": "k_merge_expand<{L}, 2, {T}>",
+    "partition": "k_partition<{L}>",
+}
+
+
+def measured_traffic(n, form, kernel, lanes, table_slots=2048):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3
+    FETCH_SIZE/WRITE_SIZE summary of the same workload (None if absent)."""
+    path = os.path.join(ROOT, "profiles", f"kernels_n{n}_{form}.txt")
+    name = ROCPROF_NAMES.get(kernel, "").format(L=lanes, T=table_slots)
+    if not name or not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic "):
+            _, k, v = line.rsplit(" ", 2)[0].split(" ", 1)[0], line[len("traffic "):].rsplit(" ", 1)[0], line.rsplit(" ", 1)[1]
+            if k == name:
+                return float(v)
+    return None
+
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
@@ -156,7 +179,8 @@ def main():
         name, s = dom
         achieved = s["algorithmic_bytes"] / s["seconds"] / 1e9
         roofline = {"bound": "hbm", "kernel": name, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                    "traffic": measured_traffic(n, args.form, name, last.crt_lanes),
                     "avg_launch_us": round(1e6 * s["seconds"] / s["launches"], 3),
                     "algorithmic_bytes_per_launch": s["algorithmic_bytes"] / s["launches"]}
 
