@@ -41,7 +41,7 @@ def measured_traffic(n, form, kernel, lanes, table_slots=2048):
         return None
     for line in open(path):
         if line.startswith("traffic "):
-            _, k, v = line.rsplit(" ", 2)[0].split(" ", 1)[0], line[len("traffic "):].rsplit(" ", 1)[0], line.rsplit(" ", 1)[1]
+            k, v = line[len("traffic "):].rstrip("\n").rsplit(" ", 1)
             if k == name:
                 return float(v)
     return None
