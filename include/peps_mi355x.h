@@ -114,7 +114,11 @@ typedef struct {
   uint64_t hbm_budget_bytes; /* arena cap (0: auto, 85% of free HBM) */
   uint32_t collect_layers;   /* nonzero: fill per-layer metrics (needs a host sync per row) */
   uint32_t expand_bits;      /* max digit width the merge-expand scatter writes (0: auto; 1..10) */
-  uint32_t reserved[6];
+  uint32_t graph_mode;       /* nonzero: record the run's size plan; repeats of the same
+                                call replay the whole row pipeline as one HIP graph */
+  uint32_t slice_group;      /* batched form: prefix tasks per device group (0: as many as
+                                the slice-id bits above the 3N-bit key allow) */
+  uint32_t reserved[4];
 } peps_options;
 
 /* ---------------------------------------------------------------- solve -- */
@@ -137,12 +141,16 @@ int peps_count_prefix(uint32_t n, const uint32_t* cols, uint32_t k, const peps_o
 
 /* Batched form: all prefix tasks of depth `split_depth` (reference
  * build_wide_crt_tasks semantics: orbit weight on row 0) contracted as
- * independent slices in grouped device launches; `out` receives the summed
- * count and per-layer totals over the slices.  `tasks_out` (may be NULL)
- * receives the number of tasks. */
+ * independent slices (slice id above the 3N-bit key) in groups of
+ * opts->slice_group tasks, each group one device pipeline (one HIP graph
+ * launch when opts->graph_mode replays a recorded plan).  A group that does
+ * not fit the HBM budget is split in two and retried.  `out` receives the
+ * summed count and per-layer totals over the slices; `tasks_out` (may be
+ * NULL) the number of tasks; task_counts[0..min(ntasks, task_cap)) (may be
+ * NULL) each task's weighted count (orbit weight x completions). */
 int peps_count_batched(uint32_t n, uint32_t split_depth, const peps_options* opts,
-                       peps_contraction_result* out, uint32_t* tasks_out, char* err,
-                       size_t err_cap);
+                       peps_contraction_result* out, uint32_t* tasks_out, peps_u128* task_counts,
+                       uint64_t task_cap, char* err, size_t err_cap);
 
 /* ------------------------------------------------------------ utilities -- */
 

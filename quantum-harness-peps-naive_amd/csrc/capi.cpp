@@ -45,6 +45,8 @@ struct Resolved {
   uint32_t expand_bits = 8;
   uint32_t device = 0;
   uint64_t budget = 0;
+  bool graph = false;
+  uint32_t slice_group = 0;
 };
 
 // Lane count: the reference plan (src/lib.rs:3859-3886) unless the caller asks
@@ -87,6 +89,8 @@ int resolve(unsigned n, const peps_options* o, Resolved* r, std::string* err) {
     }
     r->device = o->device;
     r->budget = o->hbm_budget_bytes;
+    r->graph = o->graph_mode != 0;
+    r->slice_group = o->slice_group;
   }
   return PEPS_OK;
 }
@@ -149,6 +153,7 @@ int run_engine(const Resolved& r, peps::RunSpec& spec, peps::RunOut* ro, std::st
   spec.radix_bits = r.radix_bits;
   spec.tile_states = r.tile_states;
   spec.expand_bits = r.expand_bits;
+  spec.use_graph = r.graph;
   spec.budget_bytes = r.budget;
   std::lock_guard<std::mutex> lk(eng->mutex());
   return eng->run(spec, ro, err);
@@ -284,16 +289,18 @@ int peps_count_prefix(uint32_t n, const uint32_t* cols, uint32_t k, const peps_o
 }
 
 int peps_count_batched(uint32_t n, uint32_t split_depth, const peps_options* opts,
-                       peps_contraction_result* out, uint32_t* tasks_out, char* err,
-                       size_t err_cap) {
+                       peps_contraction_result* out, uint32_t* tasks_out, peps_u128* task_counts,
+                       uint64_t task_cap, char* err, size_t err_cap) {
   if (!out) return fail(PEPS_E_INVALID, "null result pointer", err, err_cap);
   std::memset(out, 0, sizeof(*out));
   out->n = n;
   std::string e;
+  if (n > 42) return fail(PEPS_E_INVALID, "the packed u128 virtual-boundary backend supports N <= 42", err, err_cap);
   if (n == 0 || split_depth == 0 || split_depth >= n) {
-    if (n > 42) return fail(PEPS_E_INVALID, "the packed u128 virtual-boundary backend supports N <= 42", err, err_cap);
     if (tasks_out) *tasks_out = 1;
-    return peps_contract_rows(n, opts, out, err, err_cap);
+    int rc = peps_contract_rows(n, opts, out, err, err_cap);
+    if (rc == PEPS_OK && task_counts && task_cap) task_counts[0] = out->count;
+    return rc;
   }
   int rc = check_board(n, &e);
   if (rc) return fail(rc, e, err, err_cap);
@@ -302,26 +309,43 @@ int peps_count_batched(uint32_t n, uint32_t split_depth, const peps_options* opt
   if (rc) return fail(rc, e, err, err_cap);
   std::vector<Task> tasks = make_tasks(n, split_depth);
   if (tasks_out) *tasks_out = (uint32_t)tasks.size();
-  const unsigned free_bits = 63 - 3 * n;
-  const size_t group = std::min<size_t>(tasks.size(), (size_t)1 << std::min(free_bits, 20u));
+  const unsigned free_bits = std::min(63u - 3 * n, 20u);
+  size_t group = std::min<size_t>(tasks.size(), (size_t)1 << free_bits);
+  if (r.slice_group) group = std::min<size_t>(group, r.slice_group);
   const auto t0 = std::chrono::steady_clock::now();
   peps::RunOut total;
   total.residues.assign(r.primes.size(), 0);
   total.peak_states = 0;
   std::vector<peps::LayerOut> layers;
-  for (size_t g0 = 0; g0 < tasks.size(); g0 += group) {
-    const size_t g1 = std::min(tasks.size(), g0 + group);
+  std::vector<u128> per_task(tasks.size(), 0);
+  // Work list of [begin, end) task ranges; a range that exceeds the HBM budget
+  // is split in two and both halves are queued again.
+  std::vector<std::pair<size_t, size_t>> work;
+  for (size_t g0 = 0; g0 < tasks.size(); g0 += group) work.emplace_back(g0, std::min(tasks.size(), g0 + group));
+  std::reverse(work.begin(), work.end());
+  while (!work.empty()) {
+    const auto [g0, g1] = work.back();
+    work.pop_back();
     peps::RunSpec spec;
     spec.n = n;
     spec.first_row = split_depth;
     unsigned bits = 0;
     while (((size_t)1 << bits) < g1 - g0) ++bits;
     spec.slice_bits = bits;
+    spec.per_slice = true;
     for (size_t i = g0; i < g1; ++i)
       spec.seeds.push_back(peps::Seed{tasks[i].key | ((uint64_t)(i - g0) << (3 * n)), tasks[i].weight});
     peps::RunOut ro;
-    rc = run_engine(r, spec, &ro, &e);
-    if (rc) return fail(rc, e, err, err_cap);
+    std::string ge;
+    rc = run_engine(r, spec, &ro, &ge);
+    if (rc == PEPS_E_MEMORY && g1 - g0 > 1) {
+      const size_t mid = g0 + (g1 - g0) / 2;
+      work.emplace_back(mid, g1);
+      work.emplace_back(g0, mid);
+      continue;
+    }
+    if (rc) return fail(rc, ge, err, err_cap);
+    for (size_t i = g0; i < g1; ++i) per_task[i] = ro.slice_counts[i - g0];
     if (!peps::checked_add(total.count, ro.count, &total.count))
       return fail(PEPS_E_OVERFLOW, "batched count overflow", err, err_cap);
     total.peak_states = std::max(total.peak_states, ro.peak_states);
@@ -348,6 +372,7 @@ int peps_count_batched(uint32_t n, uint32_t split_depth, const peps_options* opt
   peps::factorial(n, &fact);
   if (total.count > fact) return fail(PEPS_E_OVERFLOW, "batched count exceeds N!", err, err_cap);
   for (size_t l = 0; l < r.primes.size(); ++l) total.residues[l] = (uint64_t)(total.count % r.primes[l]);
+  for (uint64_t i = 0; task_counts && i < task_cap && i < tasks.size(); ++i) task_counts[i] = abi(per_task[i]);
   const double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   fill_result(n, total, r.primes, el, out);
   return PEPS_OK;

@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <map>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -36,6 +37,8 @@ struct RunSpec {
   uint32_t radix_bits = 8;         // max hash-radix digit width per partition level
   uint32_t expand_bits = 10;       // max digit width written by the merge-expand scatter
   uint64_t budget_bytes = 0;       // arena cap (0: 85% of free HBM at first use)
+  bool per_slice = false;          // report a count per slice id
+  bool use_graph = false;          // record the size plan; replay repeats as one HIP graph
 };
 
 struct LayerOut {
@@ -61,6 +64,32 @@ struct RunOut {
   u128 total_candidates = 0, total_matched = 0;
   double elapsed_s = 0, kernel_s = 0;
   uint64_t device_peak_bytes = 0;
+  bool replayed = false;           // the run was a HIP-graph replay of a recorded plan
+};
+
+// Data-dependent sizes of one run, recorded by an eager run (which reads them
+// back from the device row by row).  The contraction is deterministic, so a
+// repeat of the same spec has the same sizes and its whole row pipeline can be
+// captured once into a HIP graph and replayed without host synchronisation.
+struct RowPlan {
+  uint64_t C_next = 0;             // candidates of the next row (0: none)
+  uint32_t W = 0;                  // merge-expand workgroups for this row
+  uint32_t d1 = 0;                 // next row's level-1 digit width
+  std::vector<unsigned> pdigits;   // next row's partition digit widths
+};
+
+struct Plan {
+  uint64_t C_first = 0;
+  u128 matched_first = 0;
+  uint32_t d1_first = 0;
+  std::vector<unsigned> pdigits_first;
+  uint64_t tiles = 1, per_tile = 1;
+  std::vector<RowPlan> rows;       // rows [first_row, n)
+  size_t region_need = 0, hist_need = 0, offsets_need = 0, partial_need = 0;
+  size_t segs_need[2] = {0, 0};
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  uint64_t generation = 0;         // buffer generation the graph was captured against
 };
 
 struct KernelStat {
@@ -91,6 +120,13 @@ class Engine {
   std::mutex mu_;
   hipStream_t stream_ = nullptr;
   hipEvent_t ev_start_ = nullptr, ev_stop_ = nullptr;
+  int pipeline(const RunSpec& spec, Plan* plan, bool eager, RunOut* out, std::string* err);
+  int finish(const RunSpec& spec, const Plan& plan, RunOut* out, std::string* err);
+  std::map<std::string, Plan> plans_;
+  uint64_t generation_ = 1;        // bumped whenever a device buffer is reallocated
+  uint64_t* slice_sums_ = nullptr;
+  size_t slice_cap_ = 0;
+
   void* region_[2] = {nullptr, nullptr};
   size_t region_bytes_[2] = {0, 0};
   uint64_t* counters_ = nullptr;   // device, kNumCounters

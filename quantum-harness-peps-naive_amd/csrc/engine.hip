@@ -110,6 +110,7 @@ int Engine::ensure_region(int idx, size_t bytes, std::string* err) {
   }
   HIP_TRY(hipMalloc(&region_[idx], want));
   region_bytes_[idx] = want;
+  ++generation_;
   return 0;
 }
 
@@ -124,6 +125,7 @@ int Engine::ensure_buffer(T** ptr, size_t* cap, size_t count, std::string* err) 
   }
   HIP_TRY(hipMalloc((void**)ptr, want * sizeof(T)));
   *cap = want;
+  ++generation_;
   return 0;
 }
 
@@ -204,6 +206,28 @@ int Engine::resolve_pending(std::string* err) {
     }                                                                   \
   } while (0)
 
+namespace {
+
+// Identity of a run for the plan cache: everything that determines sizes.
+std::string plan_key(const RunSpec& s) {
+  uint64_t h = 1469598103934665603ull;  // FNV-1a over the seeds
+  for (const Seed& sd : s.seeds) {
+    for (uint64_t v : {sd.key, (uint64_t)sd.weight, (uint64_t)(sd.weight >> 64)}) {
+      for (int b = 0; b < 8; ++b) {
+        h ^= (v >> (8 * b)) & 0xff;
+        h *= 1099511628211ull;
+      }
+    }
+  }
+  char buf[256];
+  snprintf(buf, sizeof(buf), "%u/%u/%d/%u/%zu/%u/%u/%u/%d/%zu/%llx", s.n, s.first_row,
+           (int)s.orbit_first_row, s.slice_bits, s.primes.size(), s.tile_states, s.radix_bits,
+           s.expand_bits, (int)s.per_slice, s.seeds.size(), (unsigned long long)h);
+  return buf;
+}
+
+}  // namespace
+
 int Engine::run(const RunSpec& spec, RunOut* out, std::string* err) {
   const auto t0 = Clock::now();
   HIP_TRY(hipSetDevice(device_));
@@ -213,7 +237,7 @@ int Engine::run(const RunSpec& spec, RunOut* out, std::string* err) {
     *err = "engine: unsupported board or lane count";
     return 1;
   }
-  if (3 * n + spec.slice_bits > 63) {
+  if (3 * n + spec.slice_bits > 63 || spec.slice_bits > 20) {
     *err = "engine: slice ids do not fit above the 3N-bit boundary key";
     return 1;
   }
@@ -230,6 +254,97 @@ int Engine::run(const RunSpec& spec, RunOut* out, std::string* err) {
       budget_ = (uint64_t)((double)(free_b + arena_bytes()) * 0.85);
     }
   }
+  out->replayed = false;
+  int rc;
+  if (spec.use_graph) {
+    const std::string key = plan_key(spec);
+    auto it = plans_.find(key);
+    if (it != plans_.end()) {
+      Plan& plan = it->second;
+      // Buffers at least as large as the recorded run; any reallocation
+      // invalidates a captured graph (it holds device pointers).
+      const size_t L8 = 8 + 4 * (size_t)L;
+      rc = ensure_region(0, plan.region_need, err);
+      if (!rc) rc = ensure_region(1, plan.region_need, err);
+      if (!rc) rc = ensure_buffer(&hist_, &hist_cap_, plan.hist_need, err);
+      if (!rc) rc = ensure_buffer(&offsets_, &offsets_cap_, plan.offsets_need, err);
+      if (!rc) rc = ensure_buffer(&partial_, &partial_cap_, plan.partial_need, err);
+      if (!rc) rc = ensure_buffer(&segs_[0], &segs_cap_[0], plan.segs_need[0], err);
+      if (!rc) rc = ensure_buffer(&segs_[1], &segs_cap_[1], plan.segs_need[1], err);
+      if (!rc) rc = ensure_buffer(&counters_rows_, &counters_rows_cap_, (size_t)(n + 1) * kNumCounters, err);
+      if (!rc) rc = ensure_buffer(&seed_, &seed_cap_, (spec.seeds.size() * L8 + 7) / 8, err);
+      if (!rc && spec.per_slice)
+        rc = ensure_buffer(&slice_sums_, &slice_cap_, ((size_t)1 << spec.slice_bits) * L, err);
+      if (rc) return rc;
+      if (plan.exec && plan.generation != generation_) {
+        HIP_TRY(hipGraphExecDestroy(plan.exec));
+        HIP_TRY(hipGraphDestroy(plan.graph));
+        plan.exec = nullptr;
+        plan.graph = nullptr;
+      }
+      if (!plan.exec) {
+        HIP_TRY(hipStreamSynchronize(stream_));
+        const bool prof = profiling_;
+        profiling_ = false;  // no event records inside a captured graph
+        HIP_TRY(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+        rc = pipeline(spec, &plan, false, out, err);
+        hipGraph_t graph = nullptr;
+        const hipError_t ec = hipStreamEndCapture(stream_, &graph);
+        profiling_ = prof;
+        if (rc) {
+          if (graph) hipGraphDestroy(graph);
+          return rc;
+        }
+        HIP_TRY(ec);
+        HIP_TRY(hipGraphInstantiate(&plan.exec, graph, nullptr, nullptr, 0));
+        plan.graph = graph;
+        plan.generation = generation_;
+      }
+      // Seeds are uploaded outside the graph, then the graph replays.
+      {
+        const uint64_t S0 = spec.seeds.size();
+        std::vector<uint64_t> host((S0 * L8 + 7) / 8, 0);
+        uint32_t* res = (uint32_t*)(host.data() + S0);
+        for (uint64_t i = 0; i < S0; ++i) {
+          host[i] = spec.seeds[i].key;
+          for (int l = 0; l < L; ++l) res[(size_t)l * S0 + i] = (uint32_t)(spec.seeds[i].weight % spec.primes[l]);
+        }
+        HIP_TRY(hipMemcpy(seed_, host.data(), S0 * L8, hipMemcpyHostToDevice));
+      }
+      HIP_TRY(hipEventRecord(ev_start_, stream_));
+      HIP_TRY(hipGraphLaunch(plan.exec, stream_));
+      HIP_TRY(hipEventRecord(ev_stop_, stream_));
+      HIP_TRY(hipEventSynchronize(ev_stop_));
+      rc = finish(spec, plan, out, err);
+      if (rc) return rc;
+      float ms = 0;
+      HIP_TRY(hipEventElapsedTime(&ms, ev_start_, ev_stop_));
+      out->kernel_s = ms / 1000.0;
+      out->replayed = true;
+      out->elapsed_s = std::chrono::duration<double>(Clock::now() - t0).count();
+      return 0;
+    }
+    Plan plan;
+    rc = pipeline(spec, &plan, true, out, err);
+    if (rc) return rc;
+    plans_[key] = std::move(plan);
+  } else {
+    Plan plan;
+    rc = pipeline(spec, &plan, true, out, err);
+    if (rc) return rc;
+  }
+  out->elapsed_s = std::chrono::duration<double>(Clock::now() - t0).count();
+  return 0;
+}
+
+// The row pipeline.  eager: sizes are read back from the device as the rows
+// run (host sync per row), buffers grow on demand, layer statistics are
+// collected per row, and every size is recorded into *plan.  !eager: the same
+// launch sequence is enqueued from *plan without any synchronisation (used
+// under HIP graph capture); statistics are read afterwards by finish().
+int Engine::pipeline(const RunSpec& spec, Plan* plan, bool eager, RunOut* out, std::string* err) {
+  const unsigned n = spec.n;
+  const int L = (int)spec.primes.size();
   const size_t rb = record_bytes(L);
   const double R = 8.0 + 4.0 * L;  // algorithmic bytes per record: key + L residues
   Primes pr{};
@@ -248,6 +363,7 @@ int Engine::run(const RunSpec& spec, RunOut* out, std::string* err) {
   g.dr_out = occupied_.diag_dr_out ? g.board_mask : 0u;
   g.dl_in = occupied_.diag_dl_in ? g.board_mask : 0u;
   g.dl_out = occupied_.diag_dl_out ? g.board_mask : 0u;
+  const uint64_t final_mask = g.board_mask;
 
   const uint32_t rbits = std::max<uint32_t>(1, std::min<uint32_t>(spec.radix_bits, kMaxDigitBits));
   // LDS merge table: 2048 slots (more workgroups per CU) for small tiles,
@@ -257,20 +373,48 @@ int Engine::run(const RunSpec& spec, RunOut* out, std::string* err) {
   auto total_bits = [&](uint64_t C) -> unsigned {
     return C <= target ? 0u : ceil_log2((C + target - 1) / target);
   };
+  auto split_bits = [&](unsigned B, unsigned d1) {
+    std::vector<unsigned> d;
+    unsigned used = d1;
+    while (used < B) {
+      const unsigned dk = std::min<unsigned>(rbits, B - used);
+      d.push_back(dk);
+      used += dk;
+    }
+    return d;
+  };
+  int rc = 0;
+  // Grow one candidate region; only ever called on a region that holds no
+  // live records (the write target of the next kernel).
+  auto grow = [&](int idx, size_t need_bytes) -> int {
+    plan->region_need = std::max(plan->region_need, need_bytes);
+    if (!eager) return 0;
+    return ensure_region(idx, need_bytes, err);
+  };
+  auto need = [&](auto** ptr, size_t* cap, size_t* rec, size_t count) -> int {
+    *rec = std::max(*rec, count);
+    if (!eager) return 0;
+    return ensure_buffer(ptr, cap, count, err);
+  };
 
-  // Per-row counters, cleared once.
+  // Per-row counters and slice sums, cleared with fill kernels.
   const size_t ncount = (size_t)(n + 1) * kNumCounters;
-  int rc = ensure_buffer(&counters_rows_, &counters_rows_cap_, ncount, err);
+  size_t dummy = 0;
+  rc = need(&counters_rows_, &counters_rows_cap_, &dummy, ncount);
   if (rc) return rc;
-  HIP_TRY(hipMemsetAsync(counters_rows_, 0, ncount * sizeof(uint64_t), stream_));
   auto row_counters = [&](unsigned row) { return counters_rows_ + (size_t)row * kNumCounters; };
+  const size_t nslice = spec.per_slice ? ((size_t)1 << spec.slice_bits) * L : 0;
+  if (nslice) {
+    rc = need(&slice_sums_, &slice_cap_, &dummy, nslice);
+    if (rc) return rc;
+  }
+  uint64_t* ssum = nslice ? slice_sums_ : nullptr;
 
-  // ---- seed frontier (SoA) and the first row's expansion
   const uint64_t S0 = spec.seeds.size();
   const size_t seed_bytes = S0 * (8 + 4 * (size_t)L);
-  rc = ensure_buffer(&seed_, &seed_cap_, (seed_bytes + 7) / 8, err);
-  if (rc) return rc;
-  {
+  if (eager) {
+    rc = ensure_buffer(&seed_, &seed_cap_, (seed_bytes + 7) / 8, err);
+    if (rc) return rc;
     std::vector<uint64_t> host((seed_bytes + 7) / 8, 0);
     uint64_t* keys = host.data();
     uint32_t* res = (uint32_t*)(keys + S0);
@@ -278,14 +422,17 @@ int Engine::run(const RunSpec& spec, RunOut* out, std::string* err) {
       keys[i] = spec.seeds[i].key;
       for (int l = 0; l < L; ++l) res[(size_t)l * S0 + i] = (uint32_t)(spec.seeds[i].weight % spec.primes[l]);
     }
-    HIP_TRY(hipMemcpyAsync(seed_, host.data(), seed_bytes, hipMemcpyHostToDevice, stream_));
+    HIP_TRY(hipStreamSynchronize(stream_));
+    HIP_TRY(hipMemcpy(seed_, host.data(), seed_bytes, hipMemcpyHostToDevice));
+    out->layers.clear();
+    out->peak_states = S0;
+    out->total_candidates = out->total_matched = 0;
+    out->device_peak_bytes = 0;
+    out->residues.assign(L, 0);
+    HIP_TRY(hipEventRecord(ev_start_, stream_));
   }
-  out->layers.clear();
-  out->peak_states = S0;
-  out->total_candidates = out->total_matched = 0;
-  out->device_peak_bytes = 0;
-  out->residues.assign(L, 0);
-  HIP_TRY(hipEventRecord(ev_start_, stream_));
+  LAUNCH("fill", 8.0 * ncount, launch_fill(counters_rows_, ncount, 0, stream_));
+  if (nslice) LAUNCH("fill", 8.0 * nslice, launch_fill(ssum, nslice, 0, stream_));
 
   const unsigned f = spec.first_row;
   const bool orbit = spec.orbit_first_row && f == 0;
@@ -299,10 +446,19 @@ int Engine::run(const RunSpec& spec, RunOut* out, std::string* err) {
     g.centre_bit = (n % 2) ? (1u << (n / 2)) : 0u;
   }
   LAUNCH("count", 8.0 * S0, launch_count(skeys, S0, g, c0, stream_));
-  HIP_TRY(hipMemcpyAsync(h_counters_, c0, sizeof(uint64_t) * kNumCounters, hipMemcpyDeviceToHost, stream_));
-  SYNC_AND_RESOLVE();
-  uint64_t C = h_counters_[kCandidates];  // candidates of the current row
-  u128 matched = orbit ? (u128)h_counters_[kMatchedSlot] : (u128)C;
+  uint64_t C;
+  u128 matched;
+  if (eager) {
+    HIP_TRY(hipMemcpyAsync(h_counters_, c0, sizeof(uint64_t) * kNumCounters, hipMemcpyDeviceToHost, stream_));
+    SYNC_AND_RESOLVE();
+    C = h_counters_[kCandidates];
+    matched = orbit ? (u128)h_counters_[kMatchedSlot] : (u128)C;
+    plan->C_first = C;
+    plan->matched_first = matched;
+  } else {
+    C = plan->C_first;
+    matched = plan->matched_first;
+  }
 
   int cur = 0;  // buffer holding the current row's final buckets
   uint64_t nseg = 1;
@@ -311,23 +467,21 @@ int Engine::run(const RunSpec& spec, RunOut* out, std::string* err) {
   uint64_t S_in = S0;
   unsigned levels = 0;
 
-  auto partition_to_final = [&](int from, unsigned d1, unsigned B, uint64_t Crow) -> int {
-    // level-1 buckets of `from` (segs_[sidx], 2^d1 of them) -> final buckets
+  auto partition_to_final = [&](int from, unsigned d1, const std::vector<unsigned>& pd, uint64_t Crow) -> int {
     cur = from;
     nseg = (uint64_t)1 << d1;
     levels = d1 ? 1 : 0;
     unsigned used = d1;
-    while (used < B) {
-      const unsigned dk = std::min<unsigned>(rbits, B - used);
+    for (unsigned dk : pd) {
       used += dk;
       const int to = cur ^ 1;
-      int e = ensure_region(to, Crow * rb, err);
-      if (e) return e;
-      e = ensure_buffer(&segs_[sidx ^ 1], &segs_cap_[sidx ^ 1], (nseg << dk) + 1, err);
+      int e = grow(to, Crow * rb);
+      if (!e) e = need(&segs_[sidx ^ 1], &segs_cap_[sidx ^ 1], &plan->segs_need[sidx ^ 1], (nseg << dk) + 1);
       if (e) return e;
       LAUNCH("partition", 2.0 * R * Crow,
              launch_partition(L, region_[cur], region_[to], segs_[sidx], (uint32_t)nseg, 64 - used, dk,
-                              segs_[sidx ^ 1], stream_));
+                              segs_[sidx ^ 1], region_bytes_[to] / rb, row_counters(0) + n * kNumCounters,
+                              stream_));
       cur = to;
       sidx ^= 1;
       nseg <<= dk;
@@ -338,20 +492,24 @@ int Engine::run(const RunSpec& spec, RunOut* out, std::string* err) {
 
   if (C) {
     const unsigned B = total_bits(C);
-    const unsigned d1 = std::min<unsigned>(B, kFineBits);
+    const unsigned d1 = eager ? std::min<unsigned>(B, kFineBits) : plan->d1_first;
+    const std::vector<unsigned> pd = eager ? split_bits(B, d1) : plan->pdigits_first;
+    uint64_t tiles = plan->tiles, per_tile = plan->per_tile;
+    if (eager) {
+      tiles = std::min<uint64_t>(1024, std::max<uint64_t>(1, (S0 + 4095) / 4096));
+      per_tile = (S0 + tiles - 1) / tiles;
+      tiles = (S0 + per_tile - 1) / per_tile;
+      plan->tiles = tiles;
+      plan->per_tile = per_tile;
+      plan->d1_first = d1;
+      plan->pdigits_first = pd;
+    }
     const uint32_t D1 = 1u << d1;
-    uint64_t tiles = std::min<uint64_t>(1024, std::max<uint64_t>(1, (S0 + 4095) / 4096));
-    const uint64_t per_tile = (S0 + tiles - 1) / tiles;
-    tiles = (S0 + per_tile - 1) / per_tile;
-    rc = ensure_region(0, C * rb, err);
-    if (rc) return rc;
-    rc = ensure_buffer(&hist_, &hist_cap_, tiles * D1, err);
-    if (rc) return rc;
-    rc = ensure_buffer(&offsets_, &offsets_cap_, tiles * D1, err);
-    if (rc) return rc;
-    rc = ensure_buffer(&partial_, &partial_cap_, (tiles * D1) / 4096 + 2, err);
-    if (rc) return rc;
-    rc = ensure_buffer(&segs_[0], &segs_cap_[0], (size_t)D1 + 1, err);
+    rc = grow(0, C * rb);
+    if (!rc) rc = need(&hist_, &hist_cap_, &plan->hist_need, tiles * D1);
+    if (!rc) rc = need(&offsets_, &offsets_cap_, &plan->offsets_need, tiles * D1);
+    if (!rc) rc = need(&partial_, &partial_cap_, &plan->partial_need, (tiles * D1) / 4096 + 2);
+    if (!rc) rc = need(&segs_[0], &segs_cap_[0], &plan->segs_need[0], (size_t)D1 + 1);
     if (rc) return rc;
     sidx = 0;
     LAUNCH("expand_hist", 8.0 * S0,
@@ -360,8 +518,9 @@ int Engine::run(const RunSpec& spec, RunOut* out, std::string* err) {
            launch_scan(hist_, D1, (uint32_t)tiles, 1, offsets_, segs_[0], partial_, c0, stream_));
     LAUNCH("expand_scatter", R * (S0 + C),
            launch_expand_scatter(L, skeys, sres, S0, S0, g, pr, d1, (uint32_t)tiles, per_tile, offsets_,
-                                 region_[0], stream_));
-    rc = partition_to_final(0, d1, B, C);
+                                 region_[0], region_bytes_[0] / rb, row_counters(0) + n * kNumCounters,
+                                 stream_));
+    rc = partition_to_final(0, d1, pd, C);
     if (rc) return rc;
   }
   g.orbit_row = 0;
@@ -370,6 +529,8 @@ int Engine::run(const RunSpec& spec, RunOut* out, std::string* err) {
 
   for (unsigned row = f; row < n; ++row) {
     const auto rt0 = Clock::now();
+    if (eager) plan->rows.emplace_back();
+    RowPlan& rp = plan->rows[row - f];
     LayerOut lo{};
     lo.row = row;
     lo.input_states = S_in;
@@ -379,103 +540,122 @@ int Engine::run(const RunSpec& spec, RunOut* out, std::string* err) {
     uint64_t* cr = row_counters(row);
     const bool last = row + 1 == n;
     uint64_t C_next = 0;
-    if (C == 0) {
-      lo.output_states = 0;
-      lo.output_weight = 0;
-    } else {
-      const uint32_t W = (uint32_t)std::min<uint64_t>(nseg, 1024);
+    if (C) {
+      const uint32_t W = eager ? (uint32_t)std::min<uint64_t>(nseg, 1024) : rp.W;
+      rp.W = W;
       if (last) {
         LAUNCH("merge_last", R * C,
                launch_merge_expand(kMergeLast, L, region_[cur], segs_[sidx], (uint32_t)nseg, W, g, pr, 0,
-                                   nullptr, nullptr, nullptr, (uint64_t)g.board_mask, cr, table_slots, stream_));
+                                   nullptr, nullptr, nullptr, 0, final_mask, ssum, cr, table_slots, stream_));
       } else {
         // COUNT: successor digits of the merged states at the fine resolution.
         const uint32_t DF = 1u << kFineBits;
-        rc = ensure_buffer(&hist_, &hist_cap_, (size_t)DF * W, err);
-        if (rc) return rc;
-        rc = ensure_buffer(&offsets_, &offsets_cap_, (size_t)DF * W, err);
-        if (rc) return rc;
-        rc = ensure_buffer(&partial_, &partial_cap_, ((size_t)DF * W) / 4096 + 2, err);
-        if (rc) return rc;
         const int nsidx = sidx ^ 1;
-        rc = ensure_buffer(&segs_[nsidx], &segs_cap_[nsidx], (size_t)DF + 1, err);
+        rc = need(&hist_, &hist_cap_, &plan->hist_need, (size_t)DF * W);
+        if (!rc) rc = need(&offsets_, &offsets_cap_, &plan->offsets_need, (size_t)DF * W);
+        if (!rc) rc = need(&partial_, &partial_cap_, &plan->partial_need, ((size_t)DF * W) / 4096 + 2);
+        if (!rc) rc = need(&segs_[nsidx], &segs_cap_[nsidx], &plan->segs_need[nsidx], (size_t)DF + 1);
         if (rc) return rc;
         uint64_t* cn = row_counters(row + 1);
         LAUNCH("merge_count", 8.0 * C,
                launch_merge_expand(kMergeCount, L, region_[cur], segs_[sidx], (uint32_t)nseg, W, g, pr,
-                                   kFineBits, hist_, nullptr, nullptr, 0, cr, table_slots, stream_));
-        LAUNCH("scan", 12.0 * DF * W,
-               launch_scan(hist_, DF, W, 1, offsets_, segs_[nsidx], partial_, cn, stream_));
-        HIP_TRY(hipMemcpyAsync(h_counters_, cn, sizeof(uint64_t) * kNumCounters, hipMemcpyDeviceToHost,
-                               stream_));
-        SYNC_AND_RESOLVE();
-        C_next = h_counters_[kCandidates];
+                                   kFineBits, hist_, nullptr, nullptr, 0, 0, nullptr, cr, table_slots, stream_));
+        LAUNCH("scan", 12.0 * DF * W, launch_scan(hist_, DF, W, 1, offsets_, segs_[nsidx], partial_, cn, stream_));
+        if (eager) {
+          HIP_TRY(hipMemcpyAsync(h_counters_, cn, sizeof(uint64_t) * kNumCounters, hipMemcpyDeviceToHost,
+                                 stream_));
+          SYNC_AND_RESOLVE();
+          C_next = h_counters_[kCandidates];
+          rp.C_next = C_next;
+        } else {
+          C_next = rp.C_next;
+        }
         if (C_next) {
-          const unsigned B = total_bits(C_next);
-          const unsigned d1 = std::min<unsigned>(B, std::min<unsigned>(kFineBits, std::max(1u, spec.expand_bits)));
-          if (d1 < kFineBits) {
-            LAUNCH("scan", 12.0 * DF * W,
-                   launch_scan(hist_, 1u << d1, W, 1u << (kFineBits - d1), offsets_, segs_[nsidx], partial_,
-                               cn, stream_));
+          unsigned d1 = rp.d1;
+          std::vector<unsigned> pd = rp.pdigits;
+          if (eager) {
+            const unsigned B = total_bits(C_next);
+            d1 = std::min<unsigned>(B, std::min<unsigned>(kFineBits, std::max(1u, spec.expand_bits)));
+            pd = split_bits(B, d1);
+            rp.d1 = d1;
+            rp.pdigits = pd;
           }
+          if (d1 < kFineBits)
+            LAUNCH("scan", 12.0 * DF * W,
+                   launch_scan(hist_, 1u << d1, W, 1u << (kFineBits - d1), offsets_, segs_[nsidx], partial_, cn,
+                               stream_));
           const int to = cur ^ 1;
-          rc = ensure_region(to, C_next * rb, err);
+          rc = grow(to, C_next * rb);
           if (rc) return rc;
           LAUNCH("merge_scatter", R * (C + C_next),
                  launch_merge_expand(kMergeScatter, L, region_[cur], segs_[sidx], (uint32_t)nseg, W, g, pr, d1,
-                                     nullptr, offsets_, region_[to], 0, cr, table_slots, stream_));
+                                     nullptr, offsets_, region_[to], region_bytes_[to] / rb, final_mask, ssum,
+                                     cr, table_slots, stream_));
           sidx = nsidx;
-          rc = partition_to_final(to, d1, B, C_next);
+          rc = partition_to_final(to, d1, pd, C_next);
           if (rc) return rc;
         } else {
           // No successor survives: merge for the statistics only.
           LAUNCH("merge_last", R * C,
                  launch_merge_expand(kMergeLast, L, region_[cur], segs_[sidx], (uint32_t)nseg, W, g, pr, 0,
-                                     nullptr, nullptr, nullptr, 0, cr, table_slots, stream_));
+                                     nullptr, nullptr, nullptr, 0, final_mask, ssum, cr, table_slots, stream_));
         }
       }
-      HIP_TRY(hipMemcpyAsync(h_counters_, cr, sizeof(uint64_t) * kNumCounters, hipMemcpyDeviceToHost, stream_));
-      SYNC_AND_RESOLVE();
-      if (h_counters_[kError]) {
-        *err = "LDS merge table overflow in row " + std::to_string(row + 1);
-        return 4;
+      if (eager) {
+        HIP_TRY(hipMemcpyAsync(h_counters_, cr, sizeof(uint64_t) * kNumCounters, hipMemcpyDeviceToHost, stream_));
+        SYNC_AND_RESOLVE();
+        if (h_counters_[kError]) {
+          *err = "LDS merge table overflow in row " + std::to_string(row + 1);
+          return 4;
+        }
+        std::vector<uint64_t> w(L);
+        for (int l = 0; l < L; ++l) {
+          w[l] = h_counters_[kWeight0 + l] % spec.primes[l];
+          final_res[l] = h_counters_[kFinal0 + l] % spec.primes[l];
+        }
+        std::string cerr;
+        if (!crt_reconstruct(w.data(), spec.primes.data(), L, &lo.output_weight, &cerr)) {
+          *err = cerr;
+          return 2;
+        }
+        if (lo.output_weight > fact) {
+          *err = "CRT reconstruction exceeds certified N! count bound after row " + std::to_string(row + 1);
+          return 2;
+        }
+        lo.output_states = h_counters_[kOutStates];
       }
-      std::vector<uint64_t> w(L);
-      for (int l = 0; l < L; ++l) {
-        w[l] = h_counters_[kWeight0 + l] % spec.primes[l];
-        if (last) final_res[l] = h_counters_[kFinal0 + l] % spec.primes[l];
-      }
-      std::string cerr;
-      if (!crt_reconstruct(w.data(), spec.primes.data(), L, &lo.output_weight, &cerr)) {
-        *err = cerr;
-        return 2;
-      }
-      if (lo.output_weight > fact) {
-        *err = "CRT reconstruction exceeds certified N! count bound after row " + std::to_string(row + 1);
-        return 2;
-      }
-      lo.output_states = h_counters_[kOutStates];
     }
-    lo.levels = C ? levels : 0;
-    lo.buckets = C ? (uint32_t)nseg : 0;
-    lo.device_bytes = arena_bytes() + hist_cap_ * 4 + offsets_cap_ * 8 + (segs_cap_[0] + segs_cap_[1]) * 8;
-    lo.elapsed_s = std::chrono::duration<double>(Clock::now() - rt0).count();
-    out->layers.push_back(lo);
-    out->total_candidates += lo.candidates;
-    out->total_matched += lo.matched;
-    out->peak_states = std::max<uint64_t>(out->peak_states, lo.output_states);
-    out->device_peak_bytes = std::max<uint64_t>(out->device_peak_bytes, lo.device_bytes);
-    S_in = lo.output_states;
+    if (eager) {
+      lo.levels = C ? levels : 0;
+      lo.buckets = C ? (uint32_t)nseg : 0;
+      lo.device_bytes = arena_bytes() + hist_cap_ * 4 + offsets_cap_ * 8 + (segs_cap_[0] + segs_cap_[1]) * 8;
+      lo.elapsed_s = std::chrono::duration<double>(Clock::now() - rt0).count();
+      out->layers.push_back(lo);
+      out->total_candidates += lo.candidates;
+      out->total_matched += lo.matched;
+      out->peak_states = std::max<uint64_t>(out->peak_states, lo.output_states);
+      out->device_peak_bytes = std::max<uint64_t>(out->device_peak_bytes, lo.device_bytes);
+      S_in = lo.output_states;
+    }
     C = C_next;
     matched = C_next;
   }
+  if (!eager) return 0;
+
   HIP_TRY(hipEventRecord(ev_stop_, stream_));
   HIP_TRY(hipEventSynchronize(ev_stop_));
   float ms = 0;
   HIP_TRY(hipEventElapsedTime(&ms, ev_start_, ev_stop_));
   out->kernel_s = ms / 1000.0;
-
-  // Final v1/v2 contraction: the last merge summed the residues of the states
+  // The partition and expansion kernels flag capacity violations in the
+  // spare counter row n.
+  std::vector<uint64_t> spare(kNumCounters);
+  HIP_TRY(hipMemcpy(spare.data(), row_counters(n), sizeof(uint64_t) * kNumCounters, hipMemcpyDeviceToHost));
+  if (spare[kError]) {
+    *err = "candidate buffer capacity exceeded";
+    return 4;
+  }
+  // Final v1/v2 contraction: the merges summed the residues of the states
   // whose columns are all set.
   u128 count = 0;
   const bool alive = !out->layers.empty() && out->layers.back().output_states > 0;
@@ -492,7 +672,111 @@ int Engine::run(const RunSpec& spec, RunOut* out, std::string* err) {
   }
   out->count = count;
   for (int l = 0; l < L; ++l) out->residues[l] = alive ? final_res[l] : 0;
-  out->elapsed_s = std::chrono::duration<double>(Clock::now() - t0).count();
+  if (nslice) {
+    std::vector<uint64_t> sums(nslice);
+    HIP_TRY(hipMemcpy(sums.data(), slice_sums_, nslice * 8, hipMemcpyDeviceToHost));
+    out->slice_counts.assign(nslice / L, 0);
+    std::vector<uint64_t> r(L);
+    for (size_t sl = 0; sl < nslice / L; ++sl) {
+      for (int l = 0; l < L; ++l) r[l] = sums[sl * L + l] % spec.primes[l];
+      std::string cerr;
+      if (!crt_reconstruct(r.data(), spec.primes.data(), L, &out->slice_counts[sl], &cerr)) {
+        *err = cerr;
+        return 2;
+      }
+    }
+  }
+  return 0;
+}
+
+// Layer statistics of a replayed run, read from the per-row counters.
+int Engine::finish(const RunSpec& spec, const Plan& plan, RunOut* out, std::string* err) {
+  const unsigned n = spec.n, f = spec.first_row;
+  const int L = (int)spec.primes.size();
+  u128 fact;
+  factorial(n, &fact);
+  std::vector<uint64_t> cnt((size_t)(n + 1) * kNumCounters);
+  HIP_TRY(hipMemcpy(cnt.data(), counters_rows_, cnt.size() * 8, hipMemcpyDeviceToHost));
+  if (cnt[(size_t)n * kNumCounters + kError]) {
+    *err = "candidate buffer capacity exceeded (stale size plan)";
+    return 4;
+  }
+  out->layers.clear();
+  out->peak_states = spec.seeds.size();
+  out->total_candidates = out->total_matched = 0;
+  out->residues.assign(L, 0);
+  uint64_t S_in = spec.seeds.size(), C = plan.C_first;
+  u128 matched = plan.matched_first;
+  std::vector<uint64_t> fin(L, 0);
+  for (unsigned row = f; row < n; ++row) {
+    const uint64_t* c = &cnt[(size_t)row * kNumCounters];
+    const RowPlan& rp = plan.rows[row - f];
+    LayerOut lo{};
+    lo.row = row;
+    lo.input_states = S_in;
+    lo.candidates = (u128)n * S_in;
+    lo.matched = matched;
+    lo.completed = C;
+    if (c[kError]) {
+      *err = "LDS merge table overflow in row " + std::to_string(row + 1);
+      return 4;
+    }
+    if (row + 1 < n && C && cnt[(size_t)(row + 1) * kNumCounters + kCandidates] != rp.C_next) {
+      *err = "replayed size plan does not match the device counts";
+      return 4;
+    }
+    if (C) {
+      std::vector<uint64_t> w(L);
+      for (int l = 0; l < L; ++l) {
+        w[l] = c[kWeight0 + l] % spec.primes[l];
+        fin[l] = c[kFinal0 + l] % spec.primes[l];
+      }
+      std::string cerr;
+      if (!crt_reconstruct(w.data(), spec.primes.data(), L, &lo.output_weight, &cerr)) {
+        *err = cerr;
+        return 2;
+      }
+      lo.output_states = c[kOutStates];
+    }
+    out->layers.push_back(lo);
+    out->total_candidates += lo.candidates;
+    out->total_matched += lo.matched;
+    out->peak_states = std::max<uint64_t>(out->peak_states, lo.output_states);
+    S_in = lo.output_states;
+    C = rp.C_next;
+    matched = rp.C_next;
+  }
+  u128 count = 0;
+  const bool alive = out->layers.back().output_states > 0;
+  if (alive) {
+    std::string cerr;
+    if (!crt_reconstruct(fin.data(), spec.primes.data(), L, &count, &cerr)) {
+      *err = cerr;
+      return 2;
+    }
+    if (count > fact) {
+      *err = "CRT reconstruction exceeds certified N! count bound";
+      return 2;
+    }
+  }
+  out->count = count;
+  for (int l = 0; l < L; ++l) out->residues[l] = alive ? fin[l] : 0;
+  out->device_peak_bytes = arena_bytes();
+  if (spec.per_slice) {
+    const size_t ns = ((size_t)1 << spec.slice_bits);
+    std::vector<uint64_t> sums(ns * L);
+    HIP_TRY(hipMemcpy(sums.data(), slice_sums_, sums.size() * 8, hipMemcpyDeviceToHost));
+    out->slice_counts.assign(ns, 0);
+    std::vector<uint64_t> r(L);
+    for (size_t sl = 0; sl < ns; ++sl) {
+      for (int l = 0; l < L; ++l) r[l] = sums[sl * L + l] % spec.primes[l];
+      std::string cerr;
+      if (!crt_reconstruct(r.data(), spec.primes.data(), L, &out->slice_counts[sl], &cerr)) {
+        *err = cerr;
+        return 2;
+      }
+    }
+  }
   return 0;
 }
 

@@ -64,10 +64,17 @@ hipError_t launch_count(const uint64_t* keys, uint64_t S, const RowGeom& g, uint
 // Seed-row expansion: histogram ([D][tiles], digit-major) then scatter.
 hipError_t launch_expand_hist(const uint64_t* keys, uint64_t S, const RowGeom& g, uint32_t digit_bits,
                               uint32_t tiles, uint64_t per_tile, uint32_t* hist, hipStream_t stream);
+// `cap` is the output capacity in records: a write at or beyond it is skipped
+// and sets bit 1 of counters[kError] (only a stale replayed size plan can
+// cause it).
 hipError_t launch_expand_scatter(int lanes, const uint64_t* keys, const uint32_t* res,
                                  uint64_t S, uint64_t res_stride, const RowGeom& g, const Primes& pr,
                                  uint32_t digit_bits, uint32_t tiles, uint64_t per_tile,
-                                 const uint64_t* offsets, void* out, hipStream_t stream);
+                                 const uint64_t* offsets, void* out, uint64_t cap,
+                                 uint64_t* counters, hipStream_t stream);
+
+// p[0..n) = v (a kernel, so it can be captured into a HIP graph).
+hipError_t launch_fill(uint64_t* p, uint64_t n, uint64_t v, hipStream_t stream);
 
 // Exclusive scan of a digit-major histogram hist[Dfine][W] coarsened by
 // `group` fine digits per output digit: offsets[d*W + w] (u64), segs[0..D]
@@ -79,19 +86,21 @@ hipError_t launch_scan(const uint32_t* hist, uint32_t D, uint32_t W, uint32_t gr
 
 hipError_t launch_partition(int lanes, const void* in, void* out, const uint64_t* segs,
                             uint32_t nseg, uint32_t shift, uint32_t digit_bits, uint64_t* subsegs,
-                            hipStream_t stream);
+                            uint64_t cap, uint64_t* counters, hipStream_t stream);
 
 // Fused merge of row r's buckets with the expansion into row r+1.
 // COUNT: hist[(1<<kFineBits)][W] of successor digits.  SCATTER: successors
-// written to out at offsets[(1<<digit_bits)][W].  LAST: no expansion;
-// final_mask selects the states summed into kFinal0.
+// written to out (capacity `cap` records) at offsets[(1<<digit_bits)][W].
+// LAST: no expansion.  Merged states with (key & final_mask) == final_mask
+// (all columns set) are summed into kFinal0 and, if slice_sums is given,
+// into slice_sums[slice * L + l] (slice = key bits above 3N).
 // table_slots selects the LDS table (2048 or 4096 slots); a bucket of up to
 // merge_cap(table_slots) records is merged in one pass.
 hipError_t launch_merge_expand(int mode, int lanes, const void* in, const uint64_t* segs,
                                uint32_t nseg, uint32_t W, const RowGeom& g, const Primes& pr,
                                uint32_t digit_bits, uint32_t* hist, const uint64_t* offsets,
-                               void* out, uint64_t final_mask, uint64_t* counters,
-                               uint32_t table_slots, hipStream_t stream);
+                               void* out, uint64_t cap, uint64_t final_mask, uint64_t* slice_sums,
+                               uint64_t* counters, uint32_t table_slots, hipStream_t stream);
 uint32_t merge_cap(uint32_t table_slots);
 
 }  // namespace peps

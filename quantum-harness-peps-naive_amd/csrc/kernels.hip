@@ -198,8 +198,10 @@ __device__ __forceinline__ void stage_put(Stage<L, DMAX>& st, uint32_t slot, con
 }
 
 // Write the `staged` records grouped by bucket, advance gpos, clear counts.
+// A position at or beyond `cap` (a stale size plan) is skipped and flagged.
 template <int L, int DMAX>
-__device__ void stage_flush(Stage<L, DMAX>& st, uint32_t staged, uint32_t D, Rec<L>* __restrict__ out) {
+__device__ void stage_flush(Stage<L, DMAX>& st, uint32_t staged, uint32_t D, Rec<L>* __restrict__ out,
+                            uint64_t cap, uint64_t* __restrict__ counters) {
   lds_barrier();
   scan_digits(st.count, st.start, D, st.wave_tot);
   for (uint32_t i = threadIdx.x; i < staged; i += kBlock)
@@ -208,7 +210,11 @@ __device__ void stage_flush(Stage<L, DMAX>& st, uint32_t staged, uint32_t D, Rec
   for (uint32_t j = threadIdx.x; j < staged; j += kBlock) {
     const uint32_t i = st.order[j];
     const uint32_t b = st.bucket[i];
-    out[st.gpos[b] + (j - st.start[b])] = st.rec[i];
+    const uint64_t pos = st.gpos[b] + (j - st.start[b]);
+    if (pos < cap)
+      out[pos] = st.rec[i];
+    else
+      atomicOr((unsigned long long*)&counters[kError], 2ull);
   }
   lds_barrier();
   for (uint32_t b = threadIdx.x; b < D; b += kBlock) {
@@ -229,6 +235,13 @@ __global__ __launch_bounds__(kBlock) void k_count(const uint64_t* __restrict__ k
     c += __popc(legal_positions(keys[i], g));
   c = block_sum_u64(c, scratch);
   if (threadIdx.x == 0 && c) atomicAdd((unsigned long long*)&counters[kCandidates], (unsigned long long)c);
+}
+
+// Zero fill (used instead of hipMemsetAsync so the row pipeline can be
+// captured into a HIP graph).
+__global__ __launch_bounds__(kBlock) void k_fill_u64(uint64_t* __restrict__ p, uint64_t n, uint64_t v) {
+  for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kBlock)
+    p[i] = v;
 }
 
 // ------------------------------------------------------------ expand hist --
@@ -265,7 +278,8 @@ template <int L>
 __global__ __launch_bounds__(kBlock) void k_expand_scatter(
     const uint64_t* __restrict__ keys, const uint32_t* __restrict__ res, uint64_t S,
     uint64_t res_stride, RowGeom g, Primes pr, uint32_t digit_bits, uint64_t per_tile,
-    const uint64_t* __restrict__ offsets, Rec<L>* __restrict__ out) {
+    const uint64_t* __restrict__ offsets, Rec<L>* __restrict__ out, uint64_t cap,
+    uint64_t* __restrict__ counters) {
   __shared__ Stage<L, kExpandDigits> st;
   const uint32_t D = 1u << digit_bits;
   for (uint32_t b = threadIdx.x; b < D; b += kBlock) {
@@ -296,7 +310,7 @@ __global__ __launch_bounds__(kBlock) void k_expand_scatter(
     uint32_t win = 0;
     while (win < total) {
       if (staged == kStage) {
-        stage_flush<L, kExpandDigits>(st, staged, D, out);
+        stage_flush<L, kExpandDigits>(st, staged, D, out, cap, counters);
         staged = 0;
       }
       const uint32_t space = kStage - staged;
@@ -321,7 +335,7 @@ __global__ __launch_bounds__(kBlock) void k_expand_scatter(
       lds_barrier();
     }
   }
-  if (staged) stage_flush<L, kExpandDigits>(st, staged, D, out);
+  if (staged) stage_flush<L, kExpandDigits>(st, staged, D, out, cap, counters);
 }
 
 // ------------------------------------------------------------------- scan --
@@ -414,7 +428,8 @@ __global__ __launch_bounds__(kBlock) void k_partition(const Rec<L>* __restrict__
                                                       const uint64_t* __restrict__ segs,
                                                       uint32_t nseg, uint32_t shift,
                                                       uint32_t digit_bits,
-                                                      uint64_t* __restrict__ subsegs) {
+                                                      uint64_t* __restrict__ subsegs, uint64_t cap,
+                                                      uint64_t* __restrict__ counters) {
   __shared__ Stage<L, kPartDigits> st;
   const uint32_t D = 1u << digit_bits;
   const uint64_t dmask = D - 1;
@@ -467,7 +482,7 @@ __global__ __launch_bounds__(kBlock) void k_partition(const Rec<L>* __restrict__
         const uint32_t slot = (uint32_t)j * kBlock + threadIdx.x;
         if (slot < nrec) stage_put<L, kPartDigits>(st, slot, cur[j], (uint32_t)((mix64(cur[j].key) >> shift) & dmask));
       }
-      stage_flush<L, kPartDigits>(st, nrec, D, out);
+      stage_flush<L, kPartDigits>(st, nrec, D, out, cap, counters);
     }
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) subsegs[(uint64_t)nseg * D] = segs[nseg];
@@ -494,8 +509,8 @@ template <int L, int MODE, int T>
 __global__ __launch_bounds__(kBlock) void k_merge_expand(
     const Rec<L>* __restrict__ in, const uint64_t* __restrict__ segs, uint32_t nseg, uint32_t per,
     RowGeom g, Primes pr, uint32_t digit_bits, uint32_t* __restrict__ hist,
-    const uint64_t* __restrict__ offsets, Rec<L>* __restrict__ out, uint64_t final_mask,
-    uint64_t* __restrict__ counters) {
+    const uint64_t* __restrict__ offsets, Rec<L>* __restrict__ out, uint64_t cap,
+    uint64_t final_mask, uint64_t* __restrict__ slice_sums, uint64_t* __restrict__ counters) {
   constexpr uint32_t kCap = (T * 5) / 8;             // records per pass
   constexpr int KB = (int)((kCap + kBlock - 1) / kBlock);  // records per thread per batch
   constexpr int LL = MODE == kMergeCount ? 1 : L;   // COUNT needs no residues
@@ -656,9 +671,17 @@ __global__ __launch_bounds__(kBlock) void k_merge_expand(
             v[l] = tres[l < LL ? l : 0][slot];
             wsum[l] += v[l];
           }
-          if (MODE == kMergeLast && (key & final_mask) == final_mask) {
+          // Complete boundaries (every column set) end here: the final v1/v2
+          // contraction sums them, per slice when slices are tracked.
+          if (final_mask && (key & final_mask) == final_mask) {
 #pragma unroll
             for (int l = 0; l < L; ++l) fsum[l] += v[l];
+            if (slice_sums) {
+              const uint64_t sl = (key & ~g.state_mask) >> (3 * g.n);
+#pragma unroll
+              for (int l = 0; l < L; ++l)
+                atomicAdd((unsigned long long*)&slice_sums[sl * L + l], (unsigned long long)v[l]);
+            }
           }
         }
         if (MODE != kMergeLast) {
@@ -677,7 +700,11 @@ __global__ __launch_bounds__(kBlock) void k_merge_expand(
               rec.key = sk;
 #pragma unroll
               for (int l = 0; l < L; ++l) rec.r[l] = v[l];
-              out[gpos[d] + rank] = rec;
+              const uint64_t pos = gpos[d] + rank;
+              if (pos < cap)
+                out[pos] = rec;
+              else
+                overflow |= 2u;
             }
           }
         }
@@ -700,13 +727,13 @@ __global__ __launch_bounds__(kBlock) void k_merge_expand(
     for (int l = 0; l < L; ++l) {
       const uint64_t w = block_sum_u64(wsum[l] % pr.p[l], scratch);
       if (threadIdx.x == 0 && w) atomicAdd((unsigned long long*)&counters[kWeight0 + l], (unsigned long long)(w % pr.p[l]));
-      if (MODE == kMergeLast) {
+      {
         const uint64_t f = block_sum_u64(fsum[l] % pr.p[l], scratch);
         if (threadIdx.x == 0 && f) atomicAdd((unsigned long long*)&counters[kFinal0 + l], (unsigned long long)(f % pr.p[l]));
       }
     }
   }
-  if (overflow) atomicOr((unsigned long long*)&counters[kError], 1ull);
+  if (overflow) atomicOr((unsigned long long*)&counters[kError], (unsigned long long)overflow);
 }
 
 uint32_t grid_for(uint64_t items, uint32_t per_block, uint32_t cap) {
@@ -746,16 +773,23 @@ hipError_t launch_expand_hist(const uint64_t* keys, uint64_t S, const RowGeom& g
 hipError_t launch_expand_scatter(int lanes, const uint64_t* keys, const uint32_t* res, uint64_t S,
                                  uint64_t res_stride, const RowGeom& g, const Primes& pr,
                                  uint32_t digit_bits, uint32_t tiles, uint64_t per_tile,
-                                 const uint64_t* offsets, void* out, hipStream_t stream) {
+                                 const uint64_t* offsets, void* out, uint64_t cap,
+                                 uint64_t* counters, hipStream_t stream) {
   if (digit_bits > kFineBits || tiles == 0) return hipErrorInvalidValue;
   dim3 grid(tiles), block(kBlock);
   switch (lanes) {
-    case 1: hipLaunchKernelGGL(k_expand_scatter<1>, grid, block, 0, stream, keys, res, S, res_stride, g, pr, digit_bits, per_tile, offsets, (Rec<1>*)out); break;
-    case 2: hipLaunchKernelGGL(k_expand_scatter<2>, grid, block, 0, stream, keys, res, S, res_stride, g, pr, digit_bits, per_tile, offsets, (Rec<2>*)out); break;
-    case 3: hipLaunchKernelGGL(k_expand_scatter<3>, grid, block, 0, stream, keys, res, S, res_stride, g, pr, digit_bits, per_tile, offsets, (Rec<3>*)out); break;
-    case 4: hipLaunchKernelGGL(k_expand_scatter<4>, grid, block, 0, stream, keys, res, S, res_stride, g, pr, digit_bits, per_tile, offsets, (Rec<4>*)out); break;
+    case 1: hipLaunchKernelGGL(k_expand_scatter<1>, grid, block, 0, stream, keys, res, S, res_stride, g, pr, digit_bits, per_tile, offsets, (Rec<1>*)out, cap, counters); break;
+    case 2: hipLaunchKernelGGL(k_expand_scatter<2>, grid, block, 0, stream, keys, res, S, res_stride, g, pr, digit_bits, per_tile, offsets, (Rec<2>*)out, cap, counters); break;
+    case 3: hipLaunchKernelGGL(k_expand_scatter<3>, grid, block, 0, stream, keys, res, S, res_stride, g, pr, digit_bits, per_tile, offsets, (Rec<3>*)out, cap, counters); break;
+    case 4: hipLaunchKernelGGL(k_expand_scatter<4>, grid, block, 0, stream, keys, res, S, res_stride, g, pr, digit_bits, per_tile, offsets, (Rec<4>*)out, cap, counters); break;
     default: return hipErrorInvalidValue;
   }
+  return hipGetLastError();
+}
+
+hipError_t launch_fill(uint64_t* p, uint64_t n, uint64_t v, hipStream_t stream) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(n, kBlock * 4, 1024)), dim3(kBlock), 0, stream, p, n, v);
   return hipGetLastError();
 }
 
@@ -774,14 +808,14 @@ hipError_t launch_scan(const uint32_t* hist, uint32_t D, uint32_t W, uint32_t gr
 
 hipError_t launch_partition(int lanes, const void* in, void* out, const uint64_t* segs,
                             uint32_t nseg, uint32_t shift, uint32_t digit_bits, uint64_t* subsegs,
-                            hipStream_t stream) {
+                            uint64_t cap, uint64_t* counters, hipStream_t stream) {
   if (digit_bits == 0 || digit_bits > kMaxDigitBits || nseg == 0) return hipErrorInvalidValue;
   dim3 grid(nseg < 8192 ? nseg : 8192), block(kBlock);
   switch (lanes) {
-    case 1: hipLaunchKernelGGL(k_partition<1>, grid, block, 0, stream, (const Rec<1>*)in, (Rec<1>*)out, segs, nseg, shift, digit_bits, subsegs); break;
-    case 2: hipLaunchKernelGGL(k_partition<2>, grid, block, 0, stream, (const Rec<2>*)in, (Rec<2>*)out, segs, nseg, shift, digit_bits, subsegs); break;
-    case 3: hipLaunchKernelGGL(k_partition<3>, grid, block, 0, stream, (const Rec<3>*)in, (Rec<3>*)out, segs, nseg, shift, digit_bits, subsegs); break;
-    case 4: hipLaunchKernelGGL(k_partition<4>, grid, block, 0, stream, (const Rec<4>*)in, (Rec<4>*)out, segs, nseg, shift, digit_bits, subsegs); break;
+    case 1: hipLaunchKernelGGL(k_partition<1>, grid, block, 0, stream, (const Rec<1>*)in, (Rec<1>*)out, segs, nseg, shift, digit_bits, subsegs, cap, counters); break;
+    case 2: hipLaunchKernelGGL(k_partition<2>, grid, block, 0, stream, (const Rec<2>*)in, (Rec<2>*)out, segs, nseg, shift, digit_bits, subsegs, cap, counters); break;
+    case 3: hipLaunchKernelGGL(k_partition<3>, grid, block, 0, stream, (const Rec<3>*)in, (Rec<3>*)out, segs, nseg, shift, digit_bits, subsegs, cap, counters); break;
+    case 4: hipLaunchKernelGGL(k_partition<4>, grid, block, 0, stream, (const Rec<4>*)in, (Rec<4>*)out, segs, nseg, shift, digit_bits, subsegs, cap, counters); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
@@ -791,14 +825,14 @@ template <int L, int T>
 static hipError_t launch_me(int mode, dim3 grid, hipStream_t stream, const void* in,
                             const uint64_t* segs, uint32_t nseg, uint32_t per, const RowGeom& g,
                             const Primes& pr, uint32_t digit_bits, uint32_t* hist,
-                            const uint64_t* offsets, void* out, uint64_t final_mask,
-                            uint64_t* counters) {
+                            const uint64_t* offsets, void* out, uint64_t cap, uint64_t final_mask,
+                            uint64_t* slice_sums, uint64_t* counters) {
   const Rec<L>* i = (const Rec<L>*)in;
   Rec<L>* o = (Rec<L>*)out;
   switch (mode) {
-    case kMergeCount: hipLaunchKernelGGL((k_merge_expand<L, kMergeCount, T>), grid, dim3(kBlock), 0, stream, i, segs, nseg, per, g, pr, digit_bits, hist, offsets, o, final_mask, counters); break;
-    case kMergeScatter: hipLaunchKernelGGL((k_merge_expand<L, kMergeScatter, T>), grid, dim3(kBlock), 0, stream, i, segs, nseg, per, g, pr, digit_bits, hist, offsets, o, final_mask, counters); break;
-    case kMergeLast: hipLaunchKernelGGL((k_merge_expand<L, kMergeLast, T>), grid, dim3(kBlock), 0, stream, i, segs, nseg, per, g, pr, digit_bits, hist, offsets, o, final_mask, counters); break;
+    case kMergeCount: hipLaunchKernelGGL((k_merge_expand<L, kMergeCount, T>), grid, dim3(kBlock), 0, stream, i, segs, nseg, per, g, pr, digit_bits, hist, offsets, o, cap, final_mask, slice_sums, counters); break;
+    case kMergeScatter: hipLaunchKernelGGL((k_merge_expand<L, kMergeScatter, T>), grid, dim3(kBlock), 0, stream, i, segs, nseg, per, g, pr, digit_bits, hist, offsets, o, cap, final_mask, slice_sums, counters); break;
+    case kMergeLast: hipLaunchKernelGGL((k_merge_expand<L, kMergeLast, T>), grid, dim3(kBlock), 0, stream, i, segs, nseg, per, g, pr, digit_bits, hist, offsets, o, cap, final_mask, slice_sums, counters); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
@@ -809,12 +843,12 @@ uint32_t merge_cap(uint32_t table_slots) { return (table_slots * 5) / 8; }
 hipError_t launch_merge_expand(int mode, int lanes, const void* in, const uint64_t* segs,
                                uint32_t nseg, uint32_t W, const RowGeom& g, const Primes& pr,
                                uint32_t digit_bits, uint32_t* hist, const uint64_t* offsets,
-                               void* out, uint64_t final_mask, uint64_t* counters,
-                               uint32_t table_slots, hipStream_t stream) {
+                               void* out, uint64_t cap, uint64_t final_mask, uint64_t* slice_sums,
+                               uint64_t* counters, uint32_t table_slots, hipStream_t stream) {
   if (nseg == 0 || W == 0 || digit_bits > kFineBits) return hipErrorInvalidValue;
   const uint32_t per = (nseg + W - 1) / W;
   dim3 grid(W);
-#define PEPS_ME(LN, TS) return launch_me<LN, TS>(mode, grid, stream, in, segs, nseg, per, g, pr, digit_bits, hist, offsets, out, final_mask, counters)
+#define PEPS_ME(LN, TS) return launch_me<LN, TS>(mode, grid, stream, in, segs, nseg, per, g, pr, digit_bits, hist, offsets, out, cap, final_mask, slice_sums, counters)
   if (table_slots == 2048) {
     switch (lanes) {
       case 1: PEPS_ME(1, 2048);

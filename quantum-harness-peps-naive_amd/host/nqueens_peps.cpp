@@ -183,7 +183,7 @@ void batched(int argc, char** argv, int i) {
   std::memset(&o, 0, sizeof(o));
   uint32_t tasks = 0;
   char err[1024] = {0};
-  if (peps_count_batched((uint32_t)n, (uint32_t)depth, &o, &r, &tasks, err, sizeof(err)) != PEPS_OK)
+  if (peps_count_batched((uint32_t)n, (uint32_t)depth, &o, &r, &tasks, nullptr, 0, err, sizeof(err)) != PEPS_OK)
     throw Error{err};
   u128 k;
   const bool has = known(n, &k);

@@ -108,7 +108,9 @@ class Options(ctypes.Structure):
         ("hbm_budget_bytes", ctypes.c_uint64),
         ("collect_layers", ctypes.c_uint32),
         ("expand_bits", ctypes.c_uint32),
-        ("reserved", ctypes.c_uint32 * 6),
+        ("graph_mode", ctypes.c_uint32),
+        ("slice_group", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32 * 4),
     ]
 
 

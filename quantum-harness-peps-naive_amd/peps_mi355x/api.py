@@ -44,6 +44,8 @@ class PepsOptions:
     device: int = 0
     hbm_budget_bytes: int = 0
     expand_bits: int = 0        # 0: default merge-expand scatter fan-out
+    graph_mode: int = 0         # 1: record size plans, replay repeats as one HIP graph
+    slice_group: int = 0        # batched form: tasks per device group (0: max)
 
     def to_c(self) -> Options:
         o = Options()
@@ -54,6 +56,8 @@ class PepsOptions:
         o.hbm_budget_bytes = self.hbm_budget_bytes
         o.collect_layers = 1
         o.expand_bits = self.expand_bits
+        o.graph_mode = self.graph_mode
+        o.slice_group = self.slice_group
         return o
 
 
@@ -78,7 +82,7 @@ def load():
     L.peps_count_prefix.argtypes = [c.c_uint32, P(c.c_uint32), c.c_uint32, P(Options), P(U128),
                                     c.c_char_p, c.c_size_t]
     L.peps_count_batched.argtypes = [c.c_uint32, c.c_uint32, P(Options), P(ContractionResult),
-                                     P(c.c_uint32), c.c_char_p, c.c_size_t]
+                                     P(c.c_uint32), P(U128), c.c_uint64, c.c_char_p, c.c_size_t]
     L.peps_known_count.argtypes = [c.c_uint32, P(U128)]
     L.peps_peak_rss_bytes.restype = c.c_uint64
     L.peps_crt_plan.argtypes = [c.c_uint32, P(c.c_uint64), P(c.c_uint32), P(U128), c.c_char_p, c.c_size_t]
@@ -142,12 +146,19 @@ def count_prefix(n: int, cols, options: PepsOptions = None) -> int:
     return out.value
 
 
-def count_batched(n: int, split_depth: int = 2, options: PepsOptions = None):
+def count_batched(n: int, split_depth: int = 2, options: PepsOptions = None, task_counts: bool = False):
+    """Prefix-sliced form.  Returns (result, ntasks) or, with task_counts,
+    (result, ntasks, [weighted count of each task])."""
     out = ContractionResult()
     tasks = ctypes.c_uint32()
     e = _err()
     ref, _keep = _opts(options)
-    _check(load().peps_count_batched(n, split_depth, ref, ctypes.byref(out), ctypes.byref(tasks), e, len(e)), e)
+    cap = 4096 if task_counts else 0
+    buf = (U128 * max(1, cap))()
+    _check(load().peps_count_batched(n, split_depth, ref, ctypes.byref(out), ctypes.byref(tasks),
+                                     buf if task_counts else None, cap, e, len(e)), e)
+    if task_counts:
+        return out, tasks.value, [buf[i].value for i in range(min(cap, tasks.value))]
     return out, tasks.value
 
 

@@ -144,3 +144,47 @@ def test_repeated_runs_identical():
 def test_device_is_gfx950():
     info = pm.device_info(0)
     assert "gfx950" in info["name"] and info["cu_count"] >= 200
+
+
+@pytest.mark.parametrize("n", [8, 12, 15])
+def test_graph_replay_matches_eager(n):
+    # The first call records the size plan (eager), later calls replay the
+    # captured HIP graph; all must agree with the oracle layer by layer.
+    o_ = pm.PepsOptions(graph_mode=1)
+    ref = o.contract_rows(n, backend=o.PARALLEL, threads=8)
+    for rep in range(3):
+        show("graph", n, rep)
+        g = pm.contract_rows(n, o_)
+        assert g.count.value == ref.count.value
+        assert tuples(g) == tuples(ref), rep
+        assert g.peak_states == ref.peak_states
+
+
+def test_graph_replay_interleaved_with_other_sizes():
+    # Replays after other calls that reallocate buffers must recapture.
+    o_ = pm.PepsOptions(graph_mode=1)
+    for n in (14, 16, 14, 10, 14):
+        g = pm.contract_rows(n, o_)
+        assert g.count.value == o.known_count(n)
+
+
+def test_batched_task_counts_match_dfs():
+    n, depth = 11, 2
+    b, ntasks, counts = pm.count_batched(n, depth, task_counts=True)
+    assert ntasks == len(counts) == len(o.prefix_tasks(n, depth))
+    tasks = [(c0, c1) for c0 in range((n + 1) // 2) for c1 in range(n)
+             if c1 != c0 and abs(c1 - c0) != 1]
+    assert len(tasks) == ntasks
+    for (c0, c1), cnt in zip(tasks, counts):
+        w = 1 if c0 == n - 1 - c0 else 2
+        assert cnt == w * o.count_prefix_dfs(n, [c0, c1]), (c0, c1)
+    assert sum(counts) == b.count.value == o.known_count(n)
+
+
+@pytest.mark.parametrize("group", [1, 5, 0])
+def test_batched_grouping_and_graph(group):
+    for graph in (0, 1):
+        for _ in range(2):
+            b, ntasks, counts = pm.count_batched(14, 2, pm.PepsOptions(slice_group=group, graph_mode=graph),
+                                                 task_counts=True)
+            assert b.count.value == 365596 and sum(counts) == 365596
