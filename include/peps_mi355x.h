@@ -100,6 +100,7 @@ typedef struct {
   uint64_t residues[PEPS_MAX_LANES];
   uint64_t device_peak_bytes;
   double kernel_s; /* device time between the first and last kernel of the solve */
+  double alloc_s;  /* host time spent growing the device arena during the call */
 } peps_contraction_result;
 
 /* Symmetry mode (reference SymmetryMode, src/lib.rs:715-719). */

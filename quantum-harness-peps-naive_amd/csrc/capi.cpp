@@ -131,6 +131,7 @@ void fill_result(unsigned n, const peps::RunOut& ro, const std::vector<uint64_t>
   }
   out->device_peak_bytes = ro.device_peak_bytes;
   out->kernel_s = ro.kernel_s;
+  out->alloc_s = ro.alloc_s;
   out->peak_rss_bytes = rss_bytes();
 }
 
@@ -352,6 +353,7 @@ int peps_count_batched(uint32_t n, uint32_t split_depth, const peps_options* opt
     total.total_candidates += ro.total_candidates;
     total.total_matched += ro.total_matched;
     total.kernel_s += ro.kernel_s;
+    total.alloc_s += ro.alloc_s;
     total.device_peak_bytes = std::max(total.device_peak_bytes, ro.device_peak_bytes);
     if (layers.empty()) {
       layers = ro.layers;

@@ -65,6 +65,7 @@ struct RunOut {
   double elapsed_s = 0, kernel_s = 0;
   uint64_t device_peak_bytes = 0;
   bool replayed = false;           // the run was a HIP-graph replay of a recorded plan
+  double alloc_s = 0;              // host time spent growing device buffers
 };
 
 // Data-dependent sizes of one run, recorded by an eager run (which reads them
@@ -124,6 +125,7 @@ class Engine {
   int finish(const RunSpec& spec, const Plan& plan, RunOut* out, std::string* err);
   std::map<std::string, Plan> plans_;
   uint64_t generation_ = 1;        // bumped whenever a device buffer is reallocated
+  double alloc_s_ = 0;             // cumulative host time in buffer growth
   uint64_t* slice_sums_ = nullptr;
   size_t slice_cap_ = 0;
 

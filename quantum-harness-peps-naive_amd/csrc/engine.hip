@@ -91,6 +91,12 @@ uint64_t Engine::arena_bytes() const {
 
 int Engine::ensure_region(int idx, size_t bytes, std::string* err) {
   if (region_bytes_[idx] >= bytes) return 0;
+  const auto ta = Clock::now();
+  struct Tally {
+    double* acc;
+    Clock::time_point t;
+    ~Tally() { *acc += std::chrono::duration<double>(Clock::now() - t).count(); }
+  } tally{&alloc_s_, ta};
   size_t want = std::max<size_t>(bytes + bytes / 4, (size_t)1 << 20);
   want = (want + 4095) & ~(size_t)4095;
   if (region_[idx]) {
@@ -230,6 +236,13 @@ std::string plan_key(const RunSpec& s) {
 
 int Engine::run(const RunSpec& spec, RunOut* out, std::string* err) {
   const auto t0 = Clock::now();
+  const double alloc0 = alloc_s_;
+  struct AllocReport {
+    RunOut* out;
+    const double* acc;
+    double base;
+    ~AllocReport() { out->alloc_s = *acc - base; }
+  } alloc_report{out, &alloc_s_, alloc0};
   HIP_TRY(hipSetDevice(device_));
   const unsigned n = spec.n;
   const int L = (int)spec.primes.size();

@@ -79,6 +79,7 @@ class ContractionResult(ctypes.Structure):
         ("residues", ctypes.c_uint64 * PEPS_MAX_LANES),
         ("device_peak_bytes", ctypes.c_uint64),
         ("kernel_s", ctypes.c_double),
+        ("alloc_s", ctypes.c_double),
     ]
 
     def layer_list(self) -> list:
