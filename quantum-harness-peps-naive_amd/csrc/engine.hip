@@ -97,7 +97,11 @@ int Engine::ensure_region(int idx, size_t bytes, std::string* err) {
     Clock::time_point t;
     ~Tally() { *acc += std::chrono::duration<double>(Clock::now() - t).count(); }
   } tally{&alloc_s_, ta};
+  // Large regions grow 4x at a time (capped at half the budget): every
+  // reallocation of tens of GB costs seconds, so a cold N = 18 run should see
+  // a handful of them, not one per row.
   size_t want = std::max<size_t>(bytes + bytes / 4, (size_t)1 << 20);
+  if (bytes > ((size_t)1 << 30)) want = std::max<size_t>(want, std::min<size_t>(bytes * 4, budget_ / 2));
   want = (want + 4095) & ~(size_t)4095;
   if (region_[idx]) {
     HIP_TRY(hipStreamSynchronize(stream_));
@@ -559,7 +563,8 @@ int Engine::pipeline(const RunSpec& spec, Plan* plan, bool eager, RunOut* out, s
       if (last) {
         LAUNCH("merge_last", R * C,
                launch_merge_expand(kMergeLast, L, region_[cur], segs_[sidx], (uint32_t)nseg, W, g, pr, 0,
-                                   nullptr, nullptr, nullptr, 0, final_mask, ssum, cr, table_slots, stream_));
+                                   nullptr, nullptr, nullptr, 0, final_mask, ssum, nslice / L, cr, table_slots,
+                                   stream_));
       } else {
         // COUNT: successor digits of the merged states at the fine resolution.
         const uint32_t DF = 1u << kFineBits;
@@ -572,7 +577,7 @@ int Engine::pipeline(const RunSpec& spec, Plan* plan, bool eager, RunOut* out, s
         uint64_t* cn = row_counters(row + 1);
         LAUNCH("merge_count", 8.0 * C,
                launch_merge_expand(kMergeCount, L, region_[cur], segs_[sidx], (uint32_t)nseg, W, g, pr,
-                                   kFineBits, hist_, nullptr, nullptr, 0, 0, nullptr, cr, table_slots, stream_));
+                                   kFineBits, hist_, nullptr, nullptr, 0, 0, nullptr, 0, cr, table_slots, stream_));
         LAUNCH("scan", 12.0 * DF * W, launch_scan(hist_, DF, W, 1, offsets_, segs_[nsidx], partial_, cn, stream_));
         if (eager) {
           HIP_TRY(hipMemcpyAsync(h_counters_, cn, sizeof(uint64_t) * kNumCounters, hipMemcpyDeviceToHost,
@@ -603,7 +608,7 @@ int Engine::pipeline(const RunSpec& spec, Plan* plan, bool eager, RunOut* out, s
           LAUNCH("merge_scatter", R * (C + C_next),
                  launch_merge_expand(kMergeScatter, L, region_[cur], segs_[sidx], (uint32_t)nseg, W, g, pr, d1,
                                      nullptr, offsets_, region_[to], region_bytes_[to] / rb, final_mask, ssum,
-                                     cr, table_slots, stream_));
+                                     nslice / L, cr, table_slots, stream_));
           sidx = nsidx;
           rc = partition_to_final(to, d1, pd, C_next);
           if (rc) return rc;
@@ -611,7 +616,8 @@ int Engine::pipeline(const RunSpec& spec, Plan* plan, bool eager, RunOut* out, s
           // No successor survives: merge for the statistics only.
           LAUNCH("merge_last", R * C,
                  launch_merge_expand(kMergeLast, L, region_[cur], segs_[sidx], (uint32_t)nseg, W, g, pr, 0,
-                                     nullptr, nullptr, nullptr, 0, final_mask, ssum, cr, table_slots, stream_));
+                                     nullptr, nullptr, nullptr, 0, final_mask, ssum, nslice / L, cr, table_slots,
+                                     stream_));
         }
       }
       if (eager) {

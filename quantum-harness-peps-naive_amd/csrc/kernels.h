@@ -100,7 +100,8 @@ hipError_t launch_merge_expand(int mode, int lanes, const void* in, const uint64
                                uint32_t nseg, uint32_t W, const RowGeom& g, const Primes& pr,
                                uint32_t digit_bits, uint32_t* hist, const uint64_t* offsets,
                                void* out, uint64_t cap, uint64_t final_mask, uint64_t* slice_sums,
-                               uint64_t* counters, uint32_t table_slots, hipStream_t stream);
+                               uint64_t nslices, uint64_t* counters, uint32_t table_slots,
+                               hipStream_t stream);
 uint32_t merge_cap(uint32_t table_slots);
 
 }  // namespace peps

@@ -510,7 +510,8 @@ __global__ __launch_bounds__(kBlock) void k_merge_expand(
     const Rec<L>* __restrict__ in, const uint64_t* __restrict__ segs, uint32_t nseg, uint32_t per,
     RowGeom g, Primes pr, uint32_t digit_bits, uint32_t* __restrict__ hist,
     const uint64_t* __restrict__ offsets, Rec<L>* __restrict__ out, uint64_t cap,
-    uint64_t final_mask, uint64_t* __restrict__ slice_sums, uint64_t* __restrict__ counters) {
+    uint64_t final_mask, uint64_t* __restrict__ slice_sums, uint64_t nslices,
+    uint64_t* __restrict__ counters) {
   constexpr uint32_t kCap = (T * 5) / 8;             // records per pass
   constexpr int KB = (int)((kCap + kBlock - 1) / kBlock);  // records per thread per batch
   constexpr int LL = MODE == kMergeCount ? 1 : L;   // COUNT needs no residues
@@ -522,6 +523,11 @@ __global__ __launch_bounds__(kBlock) void k_merge_expand(
   __shared__ uint64_t gpos[MODE == kMergeScatter ? DM : 1];
   __shared__ uint64_t scratch[kWaves];
   __shared__ uint32_t nuniq[2];
+  // Last row: per-slice sums gathered in LDS, flushed with one global atomic
+  // per (slice, lane) and workgroup.
+  constexpr uint32_t kLdsSlices = MODE == kMergeLast ? 1024u : 1u;
+  __shared__ uint64_t ssl[kLdsSlices * L];
+  const bool lds_slices = MODE == kMergeLast && slice_sums && nslices <= kLdsSlices;
   const uint32_t D = MODE == kMergeLast ? 1u : (1u << digit_bits);
   const uint32_t W = gridDim.x;
   const int lane = threadIdx.x & 63;
@@ -534,6 +540,8 @@ __global__ __launch_bounds__(kBlock) void k_merge_expand(
     }
   }
   if (threadIdx.x < 2) nuniq[threadIdx.x] = 0;
+  if (lds_slices)
+    for (uint32_t i = threadIdx.x; i < nslices * L; i += kBlock) ssl[i] = 0;
   lds_barrier();
 
   uint64_t uniq = 0;
@@ -679,8 +687,12 @@ __global__ __launch_bounds__(kBlock) void k_merge_expand(
             if (slice_sums) {
               const uint64_t sl = (key & ~g.state_mask) >> (3 * g.n);
 #pragma unroll
-              for (int l = 0; l < L; ++l)
-                atomicAdd((unsigned long long*)&slice_sums[sl * L + l], (unsigned long long)v[l]);
+              for (int l = 0; l < L; ++l) {
+                if (lds_slices)
+                  atomicAdd((unsigned long long*)&ssl[sl * L + l], (unsigned long long)v[l]);
+                else
+                  atomicAdd((unsigned long long*)&slice_sums[sl * L + l], (unsigned long long)v[l]);
+              }
             }
           }
         }
@@ -732,6 +744,11 @@ __global__ __launch_bounds__(kBlock) void k_merge_expand(
         if (threadIdx.x == 0 && f) atomicAdd((unsigned long long*)&counters[kFinal0 + l], (unsigned long long)(f % pr.p[l]));
       }
     }
+  }
+  if (lds_slices) {
+    lds_barrier();
+    for (uint32_t i = threadIdx.x; i < nslices * L; i += kBlock)
+      if (ssl[i]) atomicAdd((unsigned long long*)&slice_sums[i], (unsigned long long)ssl[i]);
   }
   if (overflow) atomicOr((unsigned long long*)&counters[kError], (unsigned long long)overflow);
 }
@@ -826,13 +843,13 @@ static hipError_t launch_me(int mode, dim3 grid, hipStream_t stream, const void*
                             const uint64_t* segs, uint32_t nseg, uint32_t per, const RowGeom& g,
                             const Primes& pr, uint32_t digit_bits, uint32_t* hist,
                             const uint64_t* offsets, void* out, uint64_t cap, uint64_t final_mask,
-                            uint64_t* slice_sums, uint64_t* counters) {
+                            uint64_t* slice_sums, uint64_t nslices, uint64_t* counters) {
   const Rec<L>* i = (const Rec<L>*)in;
   Rec<L>* o = (Rec<L>*)out;
   switch (mode) {
-    case kMergeCount: hipLaunchKernelGGL((k_merge_expand<L, kMergeCount, T>), grid, dim3(kBlock), 0, stream, i, segs, nseg, per, g, pr, digit_bits, hist, offsets, o, cap, final_mask, slice_sums, counters); break;
-    case kMergeScatter: hipLaunchKernelGGL((k_merge_expand<L, kMergeScatter, T>), grid, dim3(kBlock), 0, stream, i, segs, nseg, per, g, pr, digit_bits, hist, offsets, o, cap, final_mask, slice_sums, counters); break;
-    case kMergeLast: hipLaunchKernelGGL((k_merge_expand<L, kMergeLast, T>), grid, dim3(kBlock), 0, stream, i, segs, nseg, per, g, pr, digit_bits, hist, offsets, o, cap, final_mask, slice_sums, counters); break;
+    case kMergeCount: hipLaunchKernelGGL((k_merge_expand<L, kMergeCount, T>), grid, dim3(kBlock), 0, stream, i, segs, nseg, per, g, pr, digit_bits, hist, offsets, o, cap, final_mask, slice_sums, nslices, counters); break;
+    case kMergeScatter: hipLaunchKernelGGL((k_merge_expand<L, kMergeScatter, T>), grid, dim3(kBlock), 0, stream, i, segs, nseg, per, g, pr, digit_bits, hist, offsets, o, cap, final_mask, slice_sums, nslices, counters); break;
+    case kMergeLast: hipLaunchKernelGGL((k_merge_expand<L, kMergeLast, T>), grid, dim3(kBlock), 0, stream, i, segs, nseg, per, g, pr, digit_bits, hist, offsets, o, cap, final_mask, slice_sums, nslices, counters); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
@@ -844,11 +861,12 @@ hipError_t launch_merge_expand(int mode, int lanes, const void* in, const uint64
                                uint32_t nseg, uint32_t W, const RowGeom& g, const Primes& pr,
                                uint32_t digit_bits, uint32_t* hist, const uint64_t* offsets,
                                void* out, uint64_t cap, uint64_t final_mask, uint64_t* slice_sums,
-                               uint64_t* counters, uint32_t table_slots, hipStream_t stream) {
+                               uint64_t nslices, uint64_t* counters, uint32_t table_slots,
+                               hipStream_t stream) {
   if (nseg == 0 || W == 0 || digit_bits > kFineBits) return hipErrorInvalidValue;
   const uint32_t per = (nseg + W - 1) / W;
   dim3 grid(W);
-#define PEPS_ME(LN, TS) return launch_me<LN, TS>(mode, grid, stream, in, segs, nseg, per, g, pr, digit_bits, hist, offsets, out, cap, final_mask, slice_sums, counters)
+#define PEPS_ME(LN, TS) return launch_me<LN, TS>(mode, grid, stream, in, segs, nseg, per, g, pr, digit_bits, hist, offsets, out, cap, final_mask, slice_sums, nslices, counters)
   if (table_slots == 2048) {
     switch (lanes) {
       case 1: PEPS_ME(1, 2048);
