@@ -153,6 +153,11 @@ int peps_count_batched(uint32_t n, uint32_t split_depth, const peps_options* opt
                        peps_contraction_result* out, uint32_t* tasks_out, peps_u128* task_counts,
                        uint64_t task_cap, char* err, size_t err_cap);
 
+/* Progress of long calls (the batched form reports after every task group):
+ * fn(user, tasks_done, tasks_total).  NULL disables.  Process-wide. */
+typedef void (*peps_progress_fn)(void* user, uint64_t done, uint64_t total);
+int peps_set_progress(peps_progress_fn fn, void* user);
+
 /* ------------------------------------------------------------ utilities -- */
 
 /* A000170 table Q(0..27); PEPS_E_UNKNOWN outside it. */

@@ -180,6 +180,23 @@ struct Task {
   uint64_t weight;
 };
 
+peps_progress_fn g_progress = nullptr;
+void* g_progress_user = nullptr;
+
+// Number of ways to place `rows` more queens below the boundary `key`: a
+// cheap proxy (a few thousand nodes) for how large a task's frontier grows.
+uint64_t extension_count(unsigned n, uint64_t cols, uint64_t dr, uint64_t dl, unsigned rows) {
+  const uint64_t bm = ((uint64_t)1 << n) - 1;
+  if (rows == 0) return 1;
+  uint64_t free = ~(cols | dr | dl) & bm, total = 0;
+  while (free) {
+    const uint64_t b = free & (~free + 1);
+    free ^= b;
+    total += extension_count(n, cols | b, ((dr | b) << 1) & bm, (dl | b) >> 1, rows - 1);
+  }
+  return total;
+}
+
 // build_wide_crt_tasks (src/lib.rs:3888-3956) for an explicit depth: expand
 // the compiled relation row by row without merging, applying the orbit weight
 // on row 0.
@@ -311,22 +328,50 @@ int peps_count_batched(uint32_t n, uint32_t split_depth, const peps_options* opt
   std::vector<Task> tasks = make_tasks(n, split_depth);
   if (tasks_out) *tasks_out = (uint32_t)tasks.size();
   const unsigned free_bits = std::min(63u - 3 * n, 20u);
-  size_t group = std::min<size_t>(tasks.size(), (size_t)1 << free_bits);
-  if (r.slice_group) group = std::min<size_t>(group, r.slice_group);
+  const size_t max_group = std::min<size_t>((size_t)1 << free_bits, r.slice_group ? r.slice_group : SIZE_MAX);
+  // Projected frontier size of each task: its 3-row extension count, scaled
+  // by the candidates-per-extension ratio measured on the groups run so far.
+  const uint64_t bm = ((uint64_t)1 << n) - 1;
+  const unsigned ext_rows = std::min(3u, n - split_depth);
+  std::vector<double> proxy(tasks.size());
+  for (size_t i = 0; i < tasks.size(); ++i)
+    proxy[i] = (double)std::max<uint64_t>(1, extension_count(n, tasks[i].key & bm, (tasks[i].key >> n) & bm,
+                                                              (tasks[i].key >> (2 * n)) & bm, ext_rows));
+  double ratio = 0;  // max candidates per proxy unit observed
   const auto t0 = std::chrono::steady_clock::now();
   peps::RunOut total;
   total.residues.assign(r.primes.size(), 0);
   total.peak_states = 0;
   std::vector<peps::LayerOut> layers;
   std::vector<u128> per_task(tasks.size(), 0);
-  // Work list of [begin, end) task ranges; a range that exceeds the HBM budget
-  // is split in two and both halves are queued again.
+  // Task ranges still to run; a range that exceeds the HBM budget is split in
+  // two and both halves are queued again.
   std::vector<std::pair<size_t, size_t>> work;
-  for (size_t g0 = 0; g0 < tasks.size(); g0 += group) work.emplace_back(g0, std::min(tasks.size(), g0 + group));
-  std::reverse(work.begin(), work.end());
-  while (!work.empty()) {
-    const auto [g0, g1] = work.back();
-    work.pop_back();
+  size_t next = 0;
+  uint64_t done = 0;
+  peps::Engine* eng = peps::Engine::get((int)r.device, &e);
+  if (!eng) return fail(PEPS_E_DEVICE, e, err, err_cap);
+  while (next < tasks.size() || !work.empty()) {
+    size_t g0, g1;
+    if (!work.empty()) {
+      g0 = work.back().first;
+      g1 = work.back().second;
+      work.pop_back();
+    } else {
+      // Admit tasks while their projected candidates fit 80% of a buffer.
+      const double cap = 0.8 * (double)eng->candidate_capacity((int)r.primes.size());
+      g0 = next;
+      g1 = next + 1;
+      double load = proxy[next] * ratio;
+      const size_t first_cap = ratio > 0 ? max_group : std::min<size_t>(max_group, 8);
+      while (g1 < tasks.size() && g1 - g0 < first_cap) {
+        const double add = proxy[g1] * ratio;
+        if (ratio > 0 && cap > 0 && load + add > cap) break;
+        load += add;
+        ++g1;
+      }
+      next = g1;
+    }
     peps::RunSpec spec;
     spec.n = n;
     spec.first_row = split_depth;
@@ -346,6 +391,11 @@ int peps_count_batched(uint32_t n, uint32_t split_depth, const peps_options* opt
       continue;
     }
     if (rc) return fail(rc, ge, err, err_cap);
+    double psum = 0;
+    for (size_t i = g0; i < g1; ++i) psum += proxy[i];
+    u128 cmax = 0;
+    for (const auto& l : ro.layers) cmax = std::max(cmax, l.completed);
+    ratio = std::max(ratio, (double)cmax / psum);
     for (size_t i = g0; i < g1; ++i) per_task[i] = ro.slice_counts[i - g0];
     if (!peps::checked_add(total.count, ro.count, &total.count))
       return fail(PEPS_E_OVERFLOW, "batched count overflow", err, err_cap);
@@ -368,6 +418,8 @@ int peps_count_batched(uint32_t n, uint32_t split_depth, const peps_options* opt
         layers[i].elapsed_s += ro.layers[i].elapsed_s;
       }
     }
+    done += g1 - g0;
+    if (g_progress) g_progress(g_progress_user, done, tasks.size());
   }
   total.layers = layers;
   u128 fact;
@@ -451,6 +503,12 @@ int peps_device_info(uint32_t device, char* name, size_t name_cap, uint32_t* cu_
 }
 
 const char* peps_version(void) { return "peps_mi355x 0.1.0 (gfx950)"; }
+
+int peps_set_progress(peps_progress_fn fn, void* user) {
+  g_progress = fn;
+  g_progress_user = user;
+  return PEPS_OK;
+}
 
 int peps_profile_kernels(uint32_t device, uint32_t enable, char* err, size_t err_cap) {
   std::string e;

@@ -97,11 +97,19 @@ int Engine::ensure_region(int idx, size_t bytes, std::string* err) {
     Clock::time_point t;
     ~Tally() { *acc += std::chrono::duration<double>(Clock::now() - t).count(); }
   } tally{&alloc_s_, ta};
-  // Large regions grow 4x at a time (capped at half the budget): every
-  // reallocation of tens of GB costs seconds, so a cold N = 18 run should see
-  // a handful of them, not one per row.
+  // Freeing and re-allocating a region of tens of GB costs seconds, so a
+  // region that must exceed 1 GiB is allocated once at its final size, half
+  // the arena budget, and never regrown.  A row that needs more than that
+  // fails with PEPS_E_MEMORY (the batched form then splits its task group).
   size_t want = std::max<size_t>(bytes + bytes / 4, (size_t)1 << 20);
-  if (bytes > ((size_t)1 << 30)) want = std::max<size_t>(want, std::min<size_t>(bytes * 4, budget_ / 2));
+  if (bytes > ((size_t)1 << 30)) {
+    want = (size_t)(budget_ / 2);
+    if (bytes > want) {
+      *err = "HBM budget exceeded: a row needs " + std::to_string(bytes) + " bytes per candidate buffer, " +
+             "the arena allows " + std::to_string(want);
+      return 4;
+    }
+  }
   want = (want + 4095) & ~(size_t)4095;
   if (region_[idx]) {
     HIP_TRY(hipStreamSynchronize(stream_));
@@ -118,10 +126,21 @@ int Engine::ensure_region(int idx, size_t bytes, std::string* err) {
       return 4;
     }
   }
-  HIP_TRY(hipMalloc(&region_[idx], want));
+  const hipError_t me = hipMalloc(&region_[idx], want);
+  if (me == hipErrorOutOfMemory) {
+    (void)hipGetLastError();
+    region_[idx] = nullptr;
+    *err = "HBM allocation of " + std::to_string(want) + " bytes failed";
+    return 4;
+  }
+  HIP_TRY(me);
   region_bytes_[idx] = want;
   ++generation_;
   return 0;
+}
+
+uint64_t Engine::candidate_capacity(int lanes) const {
+  return budget_ ? (budget_ / 2) / record_bytes(lanes) : 0;
 }
 
 template <typename T>

@@ -173,7 +173,13 @@ void bench(int argc, char** argv, int i) {
   }
 }
 
+void progress(void*, uint64_t done, uint64_t total) {
+  std::fprintf(stderr, "batched: %llu/%llu tasks\n", (unsigned long long)done, (unsigned long long)total);
+  std::fflush(stderr);
+}
+
 void batched(int argc, char** argv, int i) {
+  peps_set_progress(progress, nullptr);
   size_t n = parse_usize(i < argc ? argv[i] : nullptr, "N");
   size_t depth = 2;
   for (int k = i + 1; k < argc; ++k)
