@@ -119,7 +119,8 @@ typedef struct {
                                 call replay the whole row pipeline as one HIP graph */
   uint32_t slice_group;      /* batched form: prefix tasks per device group (0: as many as
                                 the slice-id bits above the 3N-bit key allow) */
-  uint32_t reserved[4];
+  uint32_t merge_passes;     /* LDS merge passes per final bucket (0: auto; 1..8) */
+  uint32_t reserved[3];
 } peps_options;
 
 /* ---------------------------------------------------------------- solve -- */

@@ -47,6 +47,7 @@ struct Resolved {
   uint64_t budget = 0;
   bool graph = false;
   uint32_t slice_group = 0;
+  uint32_t merge_passes = 1;
 };
 
 // Lane count: the reference plan (src/lib.rs:3859-3886) unless the caller asks
@@ -91,6 +92,13 @@ int resolve(unsigned n, const peps_options* o, Resolved* r, std::string* err) {
     r->budget = o->hbm_budget_bytes;
     r->graph = o->graph_mode != 0;
     r->slice_group = o->slice_group;
+    if (o->merge_passes) {
+      if (o->merge_passes > 8) {
+        *err = "merge_passes must be in 1..8";
+        return PEPS_E_INVALID;
+      }
+      r->merge_passes = o->merge_passes;
+    }
   }
   return PEPS_OK;
 }
@@ -155,6 +163,7 @@ int run_engine(const Resolved& r, peps::RunSpec& spec, peps::RunOut* ro, std::st
   spec.tile_states = r.tile_states;
   spec.expand_bits = r.expand_bits;
   spec.use_graph = r.graph;
+  spec.merge_passes = r.merge_passes;
   spec.budget_bytes = r.budget;
   std::lock_guard<std::mutex> lk(eng->mutex());
   return eng->run(spec, ro, err);

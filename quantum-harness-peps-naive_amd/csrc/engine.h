@@ -36,6 +36,7 @@ struct RunSpec {
   uint32_t tile_states = 2048;     // target records per LDS merge bucket
   uint32_t radix_bits = 8;         // max hash-radix digit width per partition level
   uint32_t expand_bits = 10;       // max digit width written by the merge-expand scatter
+  uint32_t merge_passes = 1;       // target bucket = merge_passes x one LDS table load
   uint64_t budget_bytes = 0;       // arena cap (0: 85% of free HBM at first use)
   bool per_slice = false;          // report a count per slice id
   bool use_graph = false;          // record the size plan; replay repeats as one HIP graph

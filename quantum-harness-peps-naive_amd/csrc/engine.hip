@@ -249,9 +249,9 @@ std::string plan_key(const RunSpec& s) {
     }
   }
   char buf[256];
-  snprintf(buf, sizeof(buf), "%u/%u/%d/%u/%zu/%u/%u/%u/%d/%zu/%llx", s.n, s.first_row,
+  snprintf(buf, sizeof(buf), "%u/%u/%d/%u/%zu/%u/%u/%u/%u/%d/%zu/%llx", s.n, s.first_row,
            (int)s.orbit_first_row, s.slice_bits, s.primes.size(), s.tile_states, s.radix_bits,
-           s.expand_bits, (int)s.per_slice, s.seeds.size(), (unsigned long long)h);
+           s.expand_bits, s.merge_passes, (int)s.per_slice, s.seeds.size(), (unsigned long long)h);
   return buf;
 }
 
@@ -405,7 +405,10 @@ int Engine::pipeline(const RunSpec& spec, Plan* plan, bool eager, RunOut* out, s
   // LDS merge table: 2048 slots (more workgroups per CU) for small tiles,
   // 4096 slots otherwise; the bucket target stays within one merge pass.
   const uint32_t table_slots = spec.tile_states <= 1024 ? 2048u : 4096u;
-  const uint64_t target = std::max<uint32_t>(256, std::min<uint32_t>(spec.tile_states, merge_cap(table_slots)));
+  // Buckets may span several LDS passes: the merge re-reads the bucket per
+  // pass (from L2/MALL), which can be cheaper than one more partition level.
+  const uint64_t target = (uint64_t)std::max<uint32_t>(256, std::min<uint32_t>(spec.tile_states, merge_cap(table_slots))) *
+                          std::max<uint32_t>(1, std::min<uint32_t>(spec.merge_passes, 8));
   auto total_bits = [&](uint64_t C) -> unsigned {
     return C <= target ? 0u : ceil_log2((C + target - 1) / target);
   };

@@ -111,7 +111,8 @@ class Options(ctypes.Structure):
         ("expand_bits", ctypes.c_uint32),
         ("graph_mode", ctypes.c_uint32),
         ("slice_group", ctypes.c_uint32),
-        ("reserved", ctypes.c_uint32 * 4),
+        ("merge_passes", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32 * 3),
     ]
 
 

@@ -46,6 +46,7 @@ class PepsOptions:
     expand_bits: int = 0        # 0: default merge-expand scatter fan-out
     graph_mode: int = 0         # 1: record size plans, replay repeats as one HIP graph
     slice_group: int = 0        # batched form: tasks per device group (0: max)
+    merge_passes: int = 0       # LDS merge passes per final bucket (0: default)
 
     def to_c(self) -> Options:
         o = Options()
@@ -58,6 +59,7 @@ class PepsOptions:
         o.expand_bits = self.expand_bits
         o.graph_mode = self.graph_mode
         o.slice_group = self.slice_group
+        o.merge_passes = self.merge_passes
         return o
 
 
