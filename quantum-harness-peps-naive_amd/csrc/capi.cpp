@@ -48,6 +48,7 @@ struct Resolved {
   bool graph = false;
   uint32_t slice_group = 0;
   uint32_t merge_passes = 1;
+  uint32_t merge_waves = 1;
 };
 
 // Lane count: the reference plan (src/lib.rs:3859-3886) unless the caller asks
@@ -98,6 +99,13 @@ int resolve(unsigned n, const peps_options* o, Resolved* r, std::string* err) {
         return PEPS_E_INVALID;
       }
       r->merge_passes = o->merge_passes;
+    }
+    if (o->merge_waves) {
+      if (o->merge_waves > 8) {
+        *err = "merge_waves must be in 1..8";
+        return PEPS_E_INVALID;
+      }
+      r->merge_waves = o->merge_waves;
     }
   }
   return PEPS_OK;
@@ -164,6 +172,7 @@ int run_engine(const Resolved& r, peps::RunSpec& spec, peps::RunOut* ro, std::st
   spec.expand_bits = r.expand_bits;
   spec.use_graph = r.graph;
   spec.merge_passes = r.merge_passes;
+  spec.merge_waves = r.merge_waves;
   spec.budget_bytes = r.budget;
   std::lock_guard<std::mutex> lk(eng->mutex());
   return eng->run(spec, ro, err);

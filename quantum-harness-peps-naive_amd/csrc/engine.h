@@ -37,6 +37,7 @@ struct RunSpec {
   uint32_t radix_bits = 8;         // max hash-radix digit width per partition level
   uint32_t expand_bits = 10;       // max digit width written by the merge-expand scatter
   uint32_t merge_passes = 1;       // target bucket = merge_passes x one LDS table load
+  uint32_t merge_waves = 1;        // merge grid = CUs x resident workgroups x merge_waves
   uint64_t budget_bytes = 0;       // arena cap (0: 85% of free HBM at first use)
   bool per_slice = false;          // report a count per slice id
   bool use_graph = false;          // record the size plan; replay repeats as one HIP graph
@@ -150,6 +151,7 @@ class Engine {
   size_t seed_cap_ = 0;
   uint64_t budget_ = 0;
   Legs occupied_{};
+  uint32_t cu_count_ = 256;
 
   // profiling
   struct Pending {

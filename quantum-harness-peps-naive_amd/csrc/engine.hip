@@ -72,6 +72,7 @@ int Engine::init(std::string* err) {
     *err = std::string("device is ") + prop.gcnArchName + ", this library is built for gfx950";
     return 3;
   }
+  cu_count_ = (uint32_t)std::max(1, prop.multiProcessorCount);
   HIP_TRY(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
   HIP_TRY(hipEventCreate(&ev_start_));
   HIP_TRY(hipEventCreate(&ev_stop_));
@@ -249,9 +250,10 @@ std::string plan_key(const RunSpec& s) {
     }
   }
   char buf[256];
-  snprintf(buf, sizeof(buf), "%u/%u/%d/%u/%zu/%u/%u/%u/%u/%d/%zu/%llx", s.n, s.first_row,
+  snprintf(buf, sizeof(buf), "%u/%u/%d/%u/%zu/%u/%u/%u/%u/%u/%d/%zu/%llx", s.n, s.first_row,
            (int)s.orbit_first_row, s.slice_bits, s.primes.size(), s.tile_states, s.radix_bits,
-           s.expand_bits, s.merge_passes, (int)s.per_slice, s.seeds.size(), (unsigned long long)h);
+           s.expand_bits, s.merge_passes, s.merge_waves, (int)s.per_slice, s.seeds.size(),
+           (unsigned long long)h);
   return buf;
 }
 
@@ -412,6 +414,10 @@ int Engine::pipeline(const RunSpec& spec, Plan* plan, bool eager, RunOut* out, s
   auto total_bits = [&](uint64_t C) -> unsigned {
     return C <= target ? 0u : ceil_log2((C + target - 1) / target);
   };
+  // One full wave of resident SCATTER workgroups (times merge_waves), so the
+  // bucket ranges finish together instead of leaving a partly occupied tail.
+  const uint64_t merge_grid = (uint64_t)cu_count_ * merge_resident_blocks(L, table_slots) *
+                              std::max<uint32_t>(1, std::min<uint32_t>(spec.merge_waves, 8));
   auto split_bits = [&](unsigned B, unsigned d1) {
     std::vector<unsigned> d;
     unsigned used = d1;
@@ -580,7 +586,7 @@ int Engine::pipeline(const RunSpec& spec, Plan* plan, bool eager, RunOut* out, s
     const bool last = row + 1 == n;
     uint64_t C_next = 0;
     if (C) {
-      const uint32_t W = eager ? (uint32_t)std::min<uint64_t>(nseg, 1024) : rp.W;
+      const uint32_t W = eager ? (uint32_t)std::min<uint64_t>(nseg, merge_grid) : rp.W;
       rp.W = W;
       if (last) {
         LAUNCH("merge_last", R * C,

@@ -103,5 +103,7 @@ hipError_t launch_merge_expand(int mode, int lanes, const void* in, const uint64
                                uint64_t nslices, uint64_t* counters, uint32_t table_slots,
                                hipStream_t stream);
 uint32_t merge_cap(uint32_t table_slots);
+// Workgroups of the SCATTER merge kernel that fit on one CU at once.
+uint32_t merge_resident_blocks(int lanes, uint32_t table_slots);
 
 }  // namespace peps

@@ -857,6 +857,30 @@ static hipError_t launch_me(int mode, dim3 grid, hipStream_t stream, const void*
 
 uint32_t merge_cap(uint32_t table_slots) { return (table_slots * 5) / 8; }
 
+uint32_t merge_resident_blocks(int lanes, uint32_t table_slots) {
+  int blocks = 0;
+  const void* f = nullptr;
+#define PEPS_KF(LN, TS) f = (const void*)&k_merge_expand<LN, kMergeScatter, TS>
+  if (table_slots == 2048) {
+    switch (lanes) {
+      case 1: PEPS_KF(1, 2048); break;
+      case 2: PEPS_KF(2, 2048); break;
+      case 3: PEPS_KF(3, 2048); break;
+      default: PEPS_KF(4, 2048); break;
+    }
+  } else {
+    switch (lanes) {
+      case 1: PEPS_KF(1, 4096); break;
+      case 2: PEPS_KF(2, 4096); break;
+      case 3: PEPS_KF(3, 4096); break;
+      default: PEPS_KF(4, 4096); break;
+    }
+  }
+#undef PEPS_KF
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, f, kBlock, 0) != hipSuccess || blocks < 1) blocks = 1;
+  return (uint32_t)blocks;
+}
+
 hipError_t launch_merge_expand(int mode, int lanes, const void* in, const uint64_t* segs,
                                uint32_t nseg, uint32_t W, const RowGeom& g, const Primes& pr,
                                uint32_t digit_bits, uint32_t* hist, const uint64_t* offsets,

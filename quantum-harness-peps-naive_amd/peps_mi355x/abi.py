@@ -112,7 +112,8 @@ class Options(ctypes.Structure):
         ("graph_mode", ctypes.c_uint32),
         ("slice_group", ctypes.c_uint32),
         ("merge_passes", ctypes.c_uint32),
-        ("reserved", ctypes.c_uint32 * 3),
+        ("merge_waves", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32 * 2),
     ]
 
 

@@ -47,6 +47,7 @@ class PepsOptions:
     graph_mode: int = 0         # 1: record size plans, replay repeats as one HIP graph
     slice_group: int = 0        # batched form: tasks per device group (0: max)
     merge_passes: int = 0       # LDS merge passes per final bucket (0: default)
+    merge_waves: int = 0        # merge grid in waves of resident workgroups (0: default)
 
     def to_c(self) -> Options:
         o = Options()
@@ -60,6 +61,7 @@ class PepsOptions:
         o.graph_mode = self.graph_mode
         o.slice_group = self.slice_group
         o.merge_passes = self.merge_passes
+        o.merge_waves = self.merge_waves
         return o
 
 
