@@ -120,7 +120,7 @@ typedef struct {
   uint32_t slice_group;      /* batched form: prefix tasks per device group (0: as many as
                                 the slice-id bits above the 3N-bit key allow) */
   uint32_t merge_passes;     /* LDS merge passes per final bucket (0: auto; 1..8) */
-  uint32_t merge_waves;      /* merge grid in waves of resident workgroups (0: auto; 1..8) */
+  uint32_t merge_waves;      /* merge grid in waves of resident workgroups (0: 4; 1..8) */
   uint32_t reserved[2];
 } peps_options;
 

@@ -48,7 +48,7 @@ struct Resolved {
   bool graph = false;
   uint32_t slice_group = 0;
   uint32_t merge_passes = 1;
-  uint32_t merge_waves = 1;
+  uint32_t merge_waves = 4;
 };
 
 // Lane count: the reference plan (src/lib.rs:3859-3886) unless the caller asks
