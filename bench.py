@@ -10,6 +10,12 @@ time is the wall time per count with the device warmed.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 18]
                        [--form plain|batched] [--no-cpu-baseline]
+       python bench.py --form serving [--requests 2000] [--rates 50,200,800]
+                       [--serve-n-min 10] [--serve-n-max 16]
+
+The serving form streams requests (N, random non-attacking prefix of 0..2
+rows) with Poisson arrivals at each rate through the request server and
+reports requests/s and p50/p90/p99 latency per rate.
 """
 import argparse
 import json
@@ -104,13 +110,70 @@ def cpu_baseline(n, budget_s):
     }
 
 
+def serve(args, pm):
+    """Open-loop request streams through the request server, one per rate."""
+    from peps_mi355x import serving as sv
+
+    info = pm.device_info(0)
+    log(f"device: {info['name']} CUs={info['cu_count']}")
+    sweep = []
+    with pm.Server() as server:
+        # warm the engine (arena, code objects) on a short stream
+        warm = sv.make_requests(args.seed + 1, 50, 1e6, args.serve_n_min, args.serve_n_max)
+        sv.run_open_loop(server, warm)
+        for rate in [float(x) for x in args.rates.split(",")]:
+            reqs = sv.make_requests(args.seed, args.requests, rate, args.serve_n_min, args.serve_n_max)
+            st0 = server.stats()
+            outs = sv.run_open_loop(server, reqs, timeout_s=1800,
+                                    progress=lambda stage, d, t: log(f"rate {rate:g}: {stage} {d}/{t}"))
+            st1 = server.stats()
+            bad = [(n, c, o.count) for (_, n, c), o in zip(reqs, outs)
+                   if o.status != 0 or (not c and o.count != pm.known_count(n))]
+            if bad:
+                raise SystemExit(f"serving: wrong or failed requests: {bad[:5]}")
+            summ = sv.summarize(outs)
+            summ.update({"offered_rate": rate, "launches": st1["launches"] - st0["launches"],
+                         "busy_s": st1["busy_s"] - st0["busy_s"]})
+            log(json.dumps(summ))
+            sweep.append(summ)
+    best = max(sweep, key=lambda x: x["requests_per_s"])
+    out = {
+        "metric": "requests/s",
+        "value": best["requests_per_s"],
+        "unit": "requests/s",
+        "n_gpus": 1,
+        "steps": len(sweep),
+        "warmup": 1,
+        "ms_per_step": 1000.0 * best["span_s"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": (f"synthetic: {args.requests} requests per rate from splitmix64 seed {args.seed}, "
+                 f"Poisson arrivals, N uniform in [{args.serve_n_min}, {args.serve_n_max}], "
+                 f"random non-attacking prefix of 0..2 rows"),
+        "config": {"workload": "exact prefix-completion counts through the request server",
+                   "requests": args.requests, "rates": args.rates,
+                   "n_range": [args.serve_n_min, args.serve_n_max]},
+        "p50_s": best["p50_s"],
+        "p99_s": best["p99_s"],
+        "sweep": sweep,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=18)
-    ap.add_argument("--form", choices=("plain", "batched"), default="plain")
+    ap.add_argument("--form", choices=("plain", "batched", "serving"), default="plain")
+    ap.add_argument("--requests", type=int, default=2000)
+    ap.add_argument("--rates", default="50,200,800")
+    ap.add_argument("--serve-n-min", type=int, default=10)
+    ap.add_argument("--serve-n-max", type=int, default=16)
+    ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--depth", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
@@ -124,6 +187,8 @@ def main():
 
     import peps_mi355x as pm
 
+    if args.form == "serving":
+        return serve(args, pm)
     n = args.n
     info = pm.device_info(0)
     log(f"device: {info['name']} CUs={info['cu_count']} HBM={info['hbm_bytes'] / 2**30:.0f} GiB")

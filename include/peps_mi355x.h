@@ -53,7 +53,8 @@ enum {
   PEPS_E_DEVICE = 3,       /* HIP runtime error or no device */
   PEPS_E_MEMORY = 4,       /* HBM budget exceeded */
   PEPS_E_UNKNOWN = 5,      /* peps_known_count: N outside the table */
-  PEPS_E_BUSY = 6          /* serving: no free slot / queue full */
+  PEPS_E_BUSY = 6,         /* serving: the request queue is full */
+  PEPS_E_PENDING = 7       /* serving: the request has not finished yet */
 };
 
 /* Rust u128 as two little-endian 64-bit halves. */
@@ -207,6 +208,70 @@ int peps_profile_kernels(uint32_t device, uint32_t enable, char* err, size_t err
 /* Copy up to `cap` kernel stats; *count receives how many kernels have stats. */
 int peps_kernel_stats(uint32_t device, peps_kernel_stat* out, uint32_t cap, uint32_t* count,
                       char* err, size_t err_cap);
+
+/* --------------------------------------------------------------- serving -- */
+
+/* A request server: callers submit (N, prefix) counting requests from any
+ * thread; one worker thread per server turns each request into prefix tasks
+ * (rows 0..D placed, D = max(k, min(2, N-1)); an empty prefix carries the
+ * row-0 orbit weights of build_wide_crt_tasks, src/lib.rs:3888-3956) and runs
+ * them as the slices of engine launches.  Each launch takes tasks of one
+ * (N, D) class from every active request of that class (continuous batching:
+ * a request that arrives while a launch runs joins the next launch of its
+ * class), admitted while the projected candidate bytes fit 80% of one HBM
+ * candidate buffer.  Frontiers stay in HBM; a launch over budget is split.
+ * The count of a request is the sum of its tasks' slice counts, so it is the
+ * same whether it ran alone or in a mix.  There is no reference counterpart:
+ * the reference computes one count per process (src/main.rs:25-70). */
+typedef struct peps_server peps_server;
+
+enum { PEPS_POLICY_SHORTEST_FIRST = 0, PEPS_POLICY_FIFO = 1 };
+
+typedef struct {
+  uint64_t ticket;
+  uint32_t n;
+  uint32_t prefix_len;
+  int32_t status;        /* PEPS_OK, PEPS_E_PENDING, or the failure code */
+  uint32_t tasks;        /* prefix tasks the request was split into */
+  uint32_t launches;     /* engine launches that carried its tasks */
+  uint32_t max_batch;    /* most requests that shared one launch with it (itself included) */
+  peps_u128 count;       /* completions of the prefix (Q(N) for an empty prefix) */
+  double submit_s;       /* seconds since the server was created */
+  double start_s;        /* first launch carrying its tasks began */
+  double finish_s;       /* result available */
+  char error[160];
+} peps_request_result;
+
+typedef struct {
+  uint64_t submitted, completed, failed, pending;
+  uint64_t launches, tasks;
+  double busy_s;         /* worker time inside engine launches */
+  double uptime_s;
+} peps_server_stats;
+
+/* opts: engine options as for peps_contract_rows (crt_lanes applies to every
+ * N, so leave it 0 for mixed sizes).  policy: which class the worker serves
+ * next (shortest projected remaining work first, or oldest request first).
+ * max_pending: submit fails with PEPS_E_BUSY beyond this many unfinished
+ * requests (0: no limit).  PEPS_E_DEVICE without a GPU. */
+int peps_server_create(const peps_options* opts, uint32_t policy, uint32_t max_pending,
+                       peps_server** out, char* err, size_t err_cap);
+
+/* Queue the count of completions of prefix cols[0..k) on an N x N board.
+ * Invalid input fails here (same texts as peps_count_prefix). */
+int peps_server_submit(peps_server* s, uint32_t n, const uint32_t* cols, uint32_t k,
+                       uint64_t* ticket, char* err, size_t err_cap);
+
+/* PEPS_E_PENDING until the request is done, then its status; fills *out. */
+int peps_server_poll(peps_server* s, uint64_t ticket, peps_request_result* out);
+
+/* Wait up to timeout_s (negative: forever) for the request; as poll. */
+int peps_server_wait(peps_server* s, uint64_t ticket, double timeout_s, peps_request_result* out);
+
+int peps_server_stats_get(peps_server* s, peps_server_stats* out);
+
+/* Finishes every queued request, stops the worker and frees the server. */
+void peps_server_destroy(peps_server* s);
 
 #ifdef __cplusplus
 }

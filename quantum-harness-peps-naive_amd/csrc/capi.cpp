@@ -15,9 +15,13 @@
 
 #include "../../include/peps_mi355x.h"
 #include "engine.h"
+#include "frontend.h"
 #include "host_math.h"
 
 using peps::u128;
+using peps::Resolved;
+using peps::check_board;
+using peps::run_engine;
 
 namespace {
 
@@ -36,79 +40,6 @@ uint64_t rss_bytes() {
   struct rusage ru;
   if (getrusage(RUSAGE_SELF, &ru) != 0) return 0;
   return (uint64_t)ru.ru_maxrss * 1024ull;
-}
-
-struct Resolved {
-  std::vector<uint64_t> primes;
-  uint32_t radix_bits = 8;
-  uint32_t tile_states = 1024;
-  uint32_t expand_bits = 8;
-  uint32_t device = 0;
-  uint64_t budget = 0;
-  bool graph = false;
-  uint32_t slice_group = 0;
-  uint32_t merge_passes = 1;
-  uint32_t merge_waves = 4;
-};
-
-// Lane count: the reference plan (src/lib.rs:3859-3886) unless the caller asks
-// for more lanes; fewer lanes than the plan would not certify exactness.
-int resolve(unsigned n, const peps_options* o, Resolved* r, std::string* err) {
-  u128 prod, fact;
-  if (!peps::crt_plan(n, &r->primes, &prod, &fact, err)) return PEPS_E_INVALID;
-  if (o) {
-    if (o->crt_lanes) {
-      if (o->crt_lanes > PEPS_MAX_LANES) {
-        *err = "crt_lanes must be at most 4";
-        return PEPS_E_INVALID;
-      }
-      if (o->crt_lanes < r->primes.size()) {
-        *err = "crt_lanes below the certified plan (product of primes must exceed N!)";
-        return PEPS_E_INVALID;
-      }
-      r->primes.assign(peps::crt_primes(), peps::crt_primes() + o->crt_lanes);
-    }
-    if (o->radix_bits) {
-      if (o->radix_bits < 1 || o->radix_bits > 11) {
-        *err = "radix_bits must be in 1..11";
-        return PEPS_E_INVALID;
-      }
-      r->radix_bits = o->radix_bits;
-    }
-    if (o->tile_states) {
-      if (o->tile_states < 256 || o->tile_states > 2560) {
-        *err = "tile_states must be in 256..2560";
-        return PEPS_E_INVALID;
-      }
-      r->tile_states = o->tile_states;
-    }
-    if (o->expand_bits) {
-      if (o->expand_bits > 10) {
-        *err = "expand_bits must be in 1..10";
-        return PEPS_E_INVALID;
-      }
-      r->expand_bits = o->expand_bits;
-    }
-    r->device = o->device;
-    r->budget = o->hbm_budget_bytes;
-    r->graph = o->graph_mode != 0;
-    r->slice_group = o->slice_group;
-    if (o->merge_passes) {
-      if (o->merge_passes > 8) {
-        *err = "merge_passes must be in 1..8";
-        return PEPS_E_INVALID;
-      }
-      r->merge_passes = o->merge_passes;
-    }
-    if (o->merge_waves) {
-      if (o->merge_waves > 8) {
-        *err = "merge_waves must be in 1..8";
-        return PEPS_E_INVALID;
-      }
-      r->merge_waves = o->merge_waves;
-    }
-  }
-  return PEPS_OK;
 }
 
 void fill_result(unsigned n, const peps::RunOut& ro, const std::vector<uint64_t>& primes,
@@ -151,99 +82,8 @@ void fill_result(unsigned n, const peps::RunOut& ro, const std::vector<uint64_t>
   out->peak_rss_bytes = rss_bytes();
 }
 
-int check_board(unsigned n, std::string* err) {
-  if (n > 42) {
-    *err = "the packed u128 virtual-boundary backend supports N <= 42";
-    return PEPS_E_INVALID;
-  }
-  if (n > PEPS_MAX_DEVICE_N) {
-    *err = "the MI355X boundary key (u64, 3N bits) supports N <= 21";
-    return PEPS_E_INVALID;
-  }
-  return PEPS_OK;
-}
-
-int run_engine(const Resolved& r, peps::RunSpec& spec, peps::RunOut* ro, std::string* err) {
-  peps::Engine* eng = peps::Engine::get((int)r.device, err);
-  if (!eng) return PEPS_E_DEVICE;
-  spec.primes = r.primes;
-  spec.radix_bits = r.radix_bits;
-  spec.tile_states = r.tile_states;
-  spec.expand_bits = r.expand_bits;
-  spec.use_graph = r.graph;
-  spec.merge_passes = r.merge_passes;
-  spec.merge_waves = r.merge_waves;
-  spec.budget_bytes = r.budget;
-  std::lock_guard<std::mutex> lk(eng->mutex());
-  return eng->run(spec, ro, err);
-}
-
-// Boundary after a k-row prefix; false if a queen attacks an earlier one.
-bool prefix_state(unsigned n, const uint32_t* cols, unsigned k, uint64_t* key) {
-  const uint64_t bm = ((uint64_t)1 << n) - 1;
-  uint64_t c = 0, dr = 0, dl = 0;
-  for (unsigned i = 0; i < k; ++i) {
-    const uint64_t b = (uint64_t)1 << cols[i];
-    if ((c | dr | dl) & b) return false;
-    c |= b;
-    dr = ((dr | b) << 1) & bm;
-    dl = (dl | b) >> 1;
-  }
-  *key = c | (dr << n) | (dl << (2 * n));
-  return true;
-}
-
-struct Task {
-  uint64_t key;
-  uint64_t weight;
-};
-
 peps_progress_fn g_progress = nullptr;
 void* g_progress_user = nullptr;
-
-// Number of ways to place `rows` more queens below the boundary `key`: a
-// cheap proxy (a few thousand nodes) for how large a task's frontier grows.
-uint64_t extension_count(unsigned n, uint64_t cols, uint64_t dr, uint64_t dl, unsigned rows) {
-  const uint64_t bm = ((uint64_t)1 << n) - 1;
-  if (rows == 0) return 1;
-  uint64_t free = ~(cols | dr | dl) & bm, total = 0;
-  while (free) {
-    const uint64_t b = free & (~free + 1);
-    free ^= b;
-    total += extension_count(n, cols | b, ((dr | b) << 1) & bm, (dl | b) >> 1, rows - 1);
-  }
-  return total;
-}
-
-// build_wide_crt_tasks (src/lib.rs:3888-3956) for an explicit depth: expand
-// the compiled relation row by row without merging, applying the orbit weight
-// on row 0.
-std::vector<Task> make_tasks(unsigned n, unsigned depth) {
-  const uint64_t bm = n ? ((uint64_t)1 << n) - 1 : 0;
-  std::vector<Task> tasks{Task{0, 1}};
-  for (unsigned d = 0; d < depth && d < n; ++d) {
-    std::vector<Task> next;
-    for (const Task& t : tasks) {
-      const uint64_t c = t.key & bm, dr = (t.key >> n) & bm, dl = (t.key >> (2 * n)) & bm;
-      uint64_t free = ~(c | dr | dl) & bm;
-      while (free) {
-        const unsigned col = (unsigned)__builtin_ctzll(free);
-        free &= free - 1;
-        const uint64_t b = (uint64_t)1 << col;
-        uint64_t w = 1;
-        if (d == 0) {
-          const unsigned mirror = n - 1 - col;
-          if (col > mirror) continue;
-          w = col == mirror ? 1 : 2;
-        }
-        const uint64_t nc = c | b, ndr = ((dr | b) << 1) & bm, ndl = (dl | b) >> 1;
-        next.push_back(Task{nc | (ndr << n) | (ndl << (2 * n)), t.weight * w});
-      }
-    }
-    tasks.swap(next);
-  }
-  return tasks;
-}
 
 }  // namespace
 
@@ -266,7 +106,7 @@ int peps_contract_rows_with_modes(uint32_t n, uint32_t symmetry, const peps_opti
   int rc = check_board(n, &e);
   if (rc) return fail(rc, e, err, err_cap);
   Resolved r;
-  rc = resolve(n, opts, &r, &e);
+  rc = peps::resolve_options(n, opts, &r, &e);
   if (rc) return fail(rc, e, err, err_cap);
   peps::RunSpec spec;
   spec.n = n;
@@ -305,13 +145,13 @@ int peps_count_prefix(uint32_t n, const uint32_t* cols, uint32_t k, const peps_o
   int rc = check_board(n, &e);
   if (rc) return fail(rc, e, err, err_cap);
   uint64_t key;
-  if (!prefix_state(n, cols, k, &key)) return PEPS_OK;  // attacking prefix: C gives 0
+  if (!peps::prefix_state(n, cols, k, &key)) return PEPS_OK;  // attacking prefix: C gives 0
   if (k == n) {
     *count = abi(1);
     return PEPS_OK;
   }
   Resolved r;
-  rc = resolve(n, opts, &r, &e);
+  rc = peps::resolve_options(n, opts, &r, &e);
   if (rc) return fail(rc, e, err, err_cap);
   peps::RunSpec spec;
   spec.n = n;
@@ -341,82 +181,34 @@ int peps_count_batched(uint32_t n, uint32_t split_depth, const peps_options* opt
   int rc = check_board(n, &e);
   if (rc) return fail(rc, e, err, err_cap);
   Resolved r;
-  rc = resolve(n, opts, &r, &e);
+  rc = peps::resolve_options(n, opts, &r, &e);
   if (rc) return fail(rc, e, err, err_cap);
-  std::vector<Task> tasks = make_tasks(n, split_depth);
+  const std::vector<peps::Task> tasks = peps::make_tasks(n, split_depth);
   if (tasks_out) *tasks_out = (uint32_t)tasks.size();
-  const unsigned free_bits = std::min(63u - 3 * n, 20u);
-  const size_t max_group = std::min<size_t>((size_t)1 << free_bits, r.slice_group ? r.slice_group : SIZE_MAX);
-  // Projected frontier size of each task: its 3-row extension count, scaled
-  // by the candidates-per-extension ratio measured on the groups run so far.
-  const uint64_t bm = ((uint64_t)1 << n) - 1;
-  const unsigned ext_rows = std::min(3u, n - split_depth);
+  const size_t max_group = peps::max_slices(n, r.slice_group);
+  // Groups are packed by projected frontier size: each task's 3-row extension
+  // count, scaled by the candidates-per-extension ratio measured on the
+  // groups run so far (the first group is kept small to calibrate it).
+  peps::Projection proj;
   std::vector<double> proxy(tasks.size());
-  for (size_t i = 0; i < tasks.size(); ++i)
-    proxy[i] = (double)std::max<uint64_t>(1, extension_count(n, tasks[i].key & bm, (tasks[i].key >> n) & bm,
-                                                              (tasks[i].key >> (2 * n)) & bm, ext_rows));
-  double ratio = 0;  // max candidates per proxy unit observed
+  for (size_t i = 0; i < tasks.size(); ++i) proxy[i] = proj.proxy(n, split_depth, tasks[i].key);
   const auto t0 = std::chrono::steady_clock::now();
   peps::RunOut total;
   total.residues.assign(r.primes.size(), 0);
   total.peak_states = 0;
   std::vector<peps::LayerOut> layers;
   std::vector<u128> per_task(tasks.size(), 0);
-  // Task ranges still to run; a range that exceeds the HBM budget is split in
-  // two and both halves are queued again.
-  std::vector<std::pair<size_t, size_t>> work;
   size_t next = 0;
   uint64_t done = 0;
+  bool overflow = false;
   peps::Engine* eng = peps::Engine::get((int)r.device, &e);
   if (!eng) return fail(PEPS_E_DEVICE, e, err, err_cap);
-  while (next < tasks.size() || !work.empty()) {
-    size_t g0, g1;
-    if (!work.empty()) {
-      g0 = work.back().first;
-      g1 = work.back().second;
-      work.pop_back();
-    } else {
-      // Admit tasks while their projected candidates fit 80% of a buffer.
-      const double cap = 0.8 * (double)eng->candidate_capacity((int)r.primes.size());
-      g0 = next;
-      g1 = next + 1;
-      double load = proxy[next] * ratio;
-      const size_t first_cap = ratio > 0 ? max_group : std::min<size_t>(max_group, 8);
-      while (g1 < tasks.size() && g1 - g0 < first_cap) {
-        const double add = proxy[g1] * ratio;
-        if (ratio > 0 && cap > 0 && load + add > cap) break;
-        load += add;
-        ++g1;
-      }
-      next = g1;
-    }
-    peps::RunSpec spec;
-    spec.n = n;
-    spec.first_row = split_depth;
-    unsigned bits = 0;
-    while (((size_t)1 << bits) < g1 - g0) ++bits;
-    spec.slice_bits = bits;
-    spec.per_slice = true;
-    for (size_t i = g0; i < g1; ++i)
-      spec.seeds.push_back(peps::Seed{tasks[i].key | ((uint64_t)(i - g0) << (3 * n)), tasks[i].weight});
-    peps::RunOut ro;
-    std::string ge;
-    rc = run_engine(r, spec, &ro, &ge);
-    if (rc == PEPS_E_MEMORY && g1 - g0 > 1) {
-      const size_t mid = g0 + (g1 - g0) / 2;
-      work.emplace_back(mid, g1);
-      work.emplace_back(g0, mid);
-      continue;
-    }
-    if (rc) return fail(rc, ge, err, err_cap);
+  auto absorb = [&](size_t g0, size_t g1, const peps::RunOut& ro) {
     double psum = 0;
     for (size_t i = g0; i < g1; ++i) psum += proxy[i];
-    u128 cmax = 0;
-    for (const auto& l : ro.layers) cmax = std::max(cmax, l.completed);
-    ratio = std::max(ratio, (double)cmax / psum);
+    proj.learn(n, split_depth, psum, ro);
     for (size_t i = g0; i < g1; ++i) per_task[i] = ro.slice_counts[i - g0];
-    if (!peps::checked_add(total.count, ro.count, &total.count))
-      return fail(PEPS_E_OVERFLOW, "batched count overflow", err, err_cap);
+    if (!peps::checked_add(total.count, ro.count, &total.count)) overflow = true;
     total.peak_states = std::max(total.peak_states, ro.peak_states);
     total.total_candidates += ro.total_candidates;
     total.total_matched += ro.total_matched;
@@ -438,6 +230,25 @@ int peps_count_batched(uint32_t n, uint32_t split_depth, const peps_options* opt
     }
     done += g1 - g0;
     if (g_progress) g_progress(g_progress_user, done, tasks.size());
+  };
+  while (next < tasks.size()) {
+    // Admit tasks while their projected candidates fit 80% of a buffer.
+    const double ratio = proj.ratio(n, split_depth);
+    const double cap = 0.8 * (double)eng->candidate_capacity((int)r.primes.size());
+    const size_t g0 = next;
+    size_t g1 = next + 1;
+    double load = proxy[g0] * ratio;
+    const size_t first_cap = ratio > 0 ? max_group : std::min<size_t>(max_group, 8);
+    while (g1 < tasks.size() && g1 - g0 < first_cap) {
+      const double add = proxy[g1] * ratio;
+      if (ratio > 0 && cap > 0 && load + add > cap) break;
+      load += add;
+      ++g1;
+    }
+    next = g1;
+    rc = peps::run_task_range(r, n, split_depth, tasks, g0, g1, absorb, &e);
+    if (rc) return fail(rc, e, err, err_cap);
+    if (overflow) return fail(PEPS_E_OVERFLOW, "batched count overflow", err, err_cap);
   }
   total.layers = layers;
   u128 fact;
@@ -487,14 +298,14 @@ int peps_prefix_tasks(uint32_t n, uint32_t depth, uint64_t* tasks, uint64_t cap,
                       char* err, size_t err_cap) {
   if (n > PEPS_MAX_DEVICE_N)
     return fail(PEPS_E_INVALID, "prefix tasks support N <= 21", err, err_cap);
-  std::vector<Task> t = make_tasks(n, depth);
+  std::vector<peps::Task> t = peps::make_tasks(n, depth);
   if (count) *count = t.size();
   const uint64_t bm = n ? ((uint64_t)1 << n) - 1 : 0;
   for (uint64_t i = 0; i < t.size() && i < cap && tasks; ++i) {
     tasks[4 * i + 0] = t[i].key & bm;
     tasks[4 * i + 1] = (t[i].key >> n) & bm;
     tasks[4 * i + 2] = (t[i].key >> (2 * n)) & bm;
-    tasks[4 * i + 3] = t[i].weight;
+    tasks[4 * i + 3] = (uint64_t)t[i].weight;
   }
   return PEPS_OK;
 }

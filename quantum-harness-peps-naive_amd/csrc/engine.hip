@@ -330,7 +330,7 @@ int Engine::run(const RunSpec& spec, RunOut* out, std::string* err) {
         const hipError_t ec = hipStreamEndCapture(stream_, &graph);
         profiling_ = prof;
         if (rc) {
-          if (graph) hipGraphDestroy(graph);
+          if (graph) (void)hipGraphDestroy(graph);
           return rc;
         }
         HIP_TRY(ec);

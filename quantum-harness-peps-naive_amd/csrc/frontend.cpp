@@ -1,0 +1,226 @@
+// frontend.cpp -- option resolution, prefix tasks and sliced launches.
+#include "frontend.h"
+
+#include <algorithm>
+
+namespace peps {
+
+int resolve_options(unsigned n, const peps_options* o, Resolved* r, std::string* err) {
+  u128 prod, fact;
+  if (!crt_plan(n, &r->primes, &prod, &fact, err)) return PEPS_E_INVALID;
+  if (!o) return PEPS_OK;
+  if (o->crt_lanes) {
+    if (o->crt_lanes > PEPS_MAX_LANES) {
+      *err = "crt_lanes must be at most 4";
+      return PEPS_E_INVALID;
+    }
+    if (o->crt_lanes < r->primes.size()) {
+      *err = "crt_lanes below the certified plan (product of primes must exceed N!)";
+      return PEPS_E_INVALID;
+    }
+    r->primes.assign(crt_primes(), crt_primes() + o->crt_lanes);
+  }
+  if (o->radix_bits) {
+    if (o->radix_bits < 1 || o->radix_bits > 11) {
+      *err = "radix_bits must be in 1..11";
+      return PEPS_E_INVALID;
+    }
+    r->radix_bits = o->radix_bits;
+  }
+  if (o->tile_states) {
+    if (o->tile_states < 256 || o->tile_states > 2560) {
+      *err = "tile_states must be in 256..2560";
+      return PEPS_E_INVALID;
+    }
+    r->tile_states = o->tile_states;
+  }
+  if (o->expand_bits) {
+    if (o->expand_bits > 10) {
+      *err = "expand_bits must be in 1..10";
+      return PEPS_E_INVALID;
+    }
+    r->expand_bits = o->expand_bits;
+  }
+  r->device = o->device;
+  r->budget = o->hbm_budget_bytes;
+  r->graph = o->graph_mode != 0;
+  r->slice_group = o->slice_group;
+  if (o->merge_passes) {
+    if (o->merge_passes > 8) {
+      *err = "merge_passes must be in 1..8";
+      return PEPS_E_INVALID;
+    }
+    r->merge_passes = o->merge_passes;
+  }
+  if (o->merge_waves) {
+    if (o->merge_waves > 8) {
+      *err = "merge_waves must be in 1..8";
+      return PEPS_E_INVALID;
+    }
+    r->merge_waves = o->merge_waves;
+  }
+  return PEPS_OK;
+}
+
+int check_board(unsigned n, std::string* err) {
+  if (n > 42) {
+    *err = "the packed u128 virtual-boundary backend supports N <= 42";
+    return PEPS_E_INVALID;
+  }
+  if (n > PEPS_MAX_DEVICE_N) {
+    *err = "the MI355X boundary key (u64, 3N bits) supports N <= 21";
+    return PEPS_E_INVALID;
+  }
+  return PEPS_OK;
+}
+
+int run_engine(const Resolved& r, RunSpec& spec, RunOut* ro, std::string* err) {
+  Engine* eng = Engine::get((int)r.device, err);
+  if (!eng) return PEPS_E_DEVICE;
+  spec.primes = r.primes;
+  spec.radix_bits = r.radix_bits;
+  spec.tile_states = r.tile_states;
+  spec.expand_bits = r.expand_bits;
+  spec.use_graph = r.graph;
+  spec.merge_passes = r.merge_passes;
+  spec.merge_waves = r.merge_waves;
+  spec.budget_bytes = r.budget;
+  std::lock_guard<std::mutex> lk(eng->mutex());
+  return eng->run(spec, ro, err);
+}
+
+bool prefix_state(unsigned n, const uint32_t* cols, unsigned k, uint64_t* key) {
+  const uint64_t bm = ((uint64_t)1 << n) - 1;
+  uint64_t c = 0, dr = 0, dl = 0;
+  for (unsigned i = 0; i < k; ++i) {
+    const uint64_t b = (uint64_t)1 << cols[i];
+    if ((c | dr | dl) & b) return false;
+    c |= b;
+    dr = ((dr | b) << 1) & bm;
+    dl = (dl | b) >> 1;
+  }
+  *key = c | (dr << n) | (dl << (2 * n));
+  return true;
+}
+
+namespace {
+
+// One row of the compiled relation (src/lib.rs:594-638) applied to a task,
+// with the row-0 orbit weights (src/lib.rs:721-749) when `orbit` is set.
+void expand_row(unsigned n, const Task& t, bool orbit, std::vector<Task>* out) {
+  const uint64_t bm = ((uint64_t)1 << n) - 1;
+  const uint64_t c = t.key & bm, dr = (t.key >> n) & bm, dl = (t.key >> (2 * n)) & bm;
+  uint64_t free = ~(c | dr | dl) & bm;
+  while (free) {
+    const unsigned col = (unsigned)__builtin_ctzll(free);
+    free &= free - 1;
+    const uint64_t b = (uint64_t)1 << col;
+    u128 w = t.weight;
+    if (orbit) {
+      const unsigned mirror = n - 1 - col;
+      if (col > mirror) continue;
+      if (col != mirror) w *= 2;
+    }
+    const uint64_t nc = c | b, ndr = ((dr | b) << 1) & bm, ndl = (dl | b) >> 1;
+    out->push_back(Task{nc | (ndr << n) | (ndl << (2 * n)), w});
+  }
+}
+
+uint64_t extensions(unsigned n, uint64_t bm, uint64_t cols, uint64_t dr, uint64_t dl, unsigned rows) {
+  if (rows == 0) return 1;
+  uint64_t free = ~(cols | dr | dl) & bm, total = 0;
+  while (free) {
+    const uint64_t b = free & (~free + 1);
+    free ^= b;
+    total += extensions(n, bm, cols | b, ((dr | b) << 1) & bm, (dl | b) >> 1, rows - 1);
+  }
+  return total;
+}
+
+}  // namespace
+
+std::vector<Task> make_tasks(unsigned n, unsigned depth) {
+  std::vector<Task> tasks{Task{0, 1}};
+  for (unsigned d = 0; d < depth && d < n; ++d) {
+    std::vector<Task> next;
+    for (const Task& t : tasks) expand_row(n, t, d == 0, &next);
+    tasks.swap(next);
+  }
+  return tasks;
+}
+
+std::vector<Task> expand_task(unsigned n, const Task& t, unsigned from, unsigned to) {
+  std::vector<Task> tasks{t};
+  for (unsigned d = from; d < to && d < n; ++d) {
+    std::vector<Task> next;
+    for (const Task& x : tasks) expand_row(n, x, false, &next);
+    tasks.swap(next);
+  }
+  return tasks;
+}
+
+uint64_t extension_count(unsigned n, uint64_t key, unsigned rows) {
+  const uint64_t bm = ((uint64_t)1 << n) - 1;
+  return extensions(n, bm, key & bm, (key >> n) & bm, (key >> (2 * n)) & bm, rows);
+}
+
+double Projection::proxy(unsigned n, unsigned depth, uint64_t key) const {
+  const unsigned rows = std::min(3u, n > depth ? n - depth : 0u);
+  return (double)std::max<uint64_t>(1, extension_count(n, key, rows));
+}
+
+double Projection::ratio(unsigned n, unsigned depth) const {
+  auto it = ratio_.find({n, depth});
+  return it == ratio_.end() ? 0.0 : it->second;
+}
+
+void Projection::learn(unsigned n, unsigned depth, double proxy_sum, const RunOut& ro) {
+  u128 cmax = 0;
+  for (const auto& l : ro.layers) cmax = std::max(cmax, l.completed);
+  if (proxy_sum <= 0) return;
+  double& r = ratio_[{n, depth}];
+  r = std::max(r, (double)cmax / proxy_sum);
+}
+
+size_t max_slices(unsigned n, uint32_t slice_group) {
+  const unsigned free_bits = std::min(63u - 3 * n, 20u);
+  return std::min<size_t>((size_t)1 << free_bits, slice_group ? slice_group : SIZE_MAX);
+}
+
+int run_task_range(const Resolved& r, unsigned n, unsigned depth, const std::vector<Task>& tasks,
+                   size_t g0, size_t g1,
+                   const std::function<void(size_t, size_t, const RunOut&)>& done,
+                   std::string* err) {
+  std::vector<std::pair<size_t, size_t>> work{{g0, g1}};
+  while (!work.empty()) {
+    const auto [a, b] = work.back();
+    work.pop_back();
+    RunSpec spec;
+    spec.n = n;
+    spec.first_row = depth;
+    spec.orbit_first_row = depth == 0;
+    unsigned bits = 0;
+    while (((size_t)1 << bits) < b - a) ++bits;
+    spec.slice_bits = bits;
+    spec.per_slice = true;
+    for (size_t i = a; i < b; ++i)
+      spec.seeds.push_back(Seed{tasks[i].key | ((uint64_t)(i - a) << (3 * n)), tasks[i].weight});
+    RunOut ro;
+    std::string e;
+    const int rc = run_engine(r, spec, &ro, &e);
+    if (rc == PEPS_E_MEMORY && b - a > 1) {
+      const size_t mid = a + (b - a) / 2;
+      work.emplace_back(mid, b);
+      work.emplace_back(a, mid);
+      continue;
+    }
+    if (rc) {
+      *err = e;
+      return rc;
+    }
+    done(a, b, ro);
+  }
+  return PEPS_OK;
+}
+
+}  // namespace peps

@@ -1,0 +1,90 @@
+// frontend.h -- host-side pieces shared by the C ABI entry points and the
+// request server: option resolution, prefix states, prefix tasks, the
+// frontier-size projection and the sliced launch of a group of tasks.
+#pragma once
+
+#include <stdint.h>
+
+#include <functional>
+#include <map>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../include/peps_mi355x.h"
+#include "engine.h"
+#include "host_math.h"
+
+namespace peps {
+
+// peps_options after validation and defaults.
+struct Resolved {
+  std::vector<uint64_t> primes;
+  uint32_t radix_bits = 8;
+  uint32_t tile_states = 1024;
+  uint32_t expand_bits = 8;
+  uint32_t device = 0;
+  uint64_t budget = 0;
+  bool graph = false;
+  uint32_t slice_group = 0;
+  uint32_t merge_passes = 1;
+  uint32_t merge_waves = 4;
+};
+
+// Lane count: the reference plan (src/lib.rs:3859-3886) unless the caller
+// asks for more lanes; fewer lanes than the plan would not certify exactness.
+int resolve_options(unsigned n, const peps_options* o, Resolved* r, std::string* err);
+
+// N > 42: the reference's text (src/lib.rs:5293); 22..42: the u64 key limit.
+int check_board(unsigned n, std::string* err);
+
+// Runs `spec` on the resolved device with the resolved tuning.
+int run_engine(const Resolved& r, RunSpec& spec, RunOut* ro, std::string* err);
+
+// Boundary after a k-row prefix; false if a queen attacks an earlier one.
+bool prefix_state(unsigned n, const uint32_t* cols, unsigned k, uint64_t* key);
+
+// One prefix task: the boundary after `depth` rows and its multiplicity.
+struct Task {
+  uint64_t key;
+  u128 weight;
+};
+
+// build_wide_crt_tasks (src/lib.rs:3888-3956) for an explicit depth: expand
+// the compiled relation row by row without merging, applying the orbit weight
+// on row 0.
+std::vector<Task> make_tasks(unsigned n, unsigned depth);
+
+// Expands one boundary at row `from` to row `to` without merging and without
+// the orbit filter (the rows above `from` are already placed).
+std::vector<Task> expand_task(unsigned n, const Task& t, unsigned from, unsigned to);
+
+// Number of ways to place `rows` more queens below a boundary: a cheap proxy
+// (a few thousand nodes) for how large a task's frontier grows.
+uint64_t extension_count(unsigned n, uint64_t key, unsigned rows);
+
+// Candidates per proxy unit, learned per (N, start row) from the launches run
+// so far (the maximum over rows of emitted candidates / Σ proxies).
+class Projection {
+ public:
+  double proxy(unsigned n, unsigned depth, uint64_t key) const;
+  double ratio(unsigned n, unsigned depth) const;  // 0: not yet calibrated
+  void learn(unsigned n, unsigned depth, double proxy_sum, const RunOut& ro);
+
+ private:
+  std::map<std::pair<unsigned, unsigned>, double> ratio_;
+};
+
+// Largest number of tasks one launch can carry as slices (ids above bit 3N).
+size_t max_slices(unsigned n, uint32_t slice_group);
+
+// Runs tasks[g0, g1) as the slices of one launch starting at row `depth`; a
+// launch that exceeds the HBM budget is split in halves.  `done` is called
+// once per successful launch with its range and output; `slice_counts` of that
+// output are per task of the range.
+int run_task_range(const Resolved& r, unsigned n, unsigned depth, const std::vector<Task>& tasks,
+                   size_t g0, size_t g1,
+                   const std::function<void(size_t, size_t, const RunOut&)>& done,
+                   std::string* err);
+
+}  // namespace peps

@@ -8,6 +8,8 @@ raise ``PepsError`` -- there is no CPU fallback.
 from .api import (  # noqa: F401
     PepsError,
     PepsOptions,
+    RequestOutcome,
+    Server,
     contract_rows,
     contract_rows_d4_orbit_sort_reduce,
     contract_rows_sort_reduce,

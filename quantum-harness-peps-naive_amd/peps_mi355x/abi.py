@@ -18,6 +18,10 @@ PEPS_E_DEVICE = 3
 PEPS_E_MEMORY = 4
 PEPS_E_UNKNOWN = 5
 PEPS_E_BUSY = 6
+PEPS_E_PENDING = 7
+
+PEPS_POLICY_SHORTEST_FIRST = 0
+PEPS_POLICY_FIFO = 1
 
 PEPS_SYMMETRY_NONE = 0
 PEPS_SYMMETRY_TOP_ROW_ORBITS = 1
@@ -123,4 +127,34 @@ class KernelStat(ctypes.Structure):
         ("launches", ctypes.c_uint64),
         ("seconds", ctypes.c_double),
         ("algorithmic_bytes", ctypes.c_double),
+    ]
+
+
+class RequestResult(ctypes.Structure):
+    _fields_ = [
+        ("ticket", ctypes.c_uint64),
+        ("n", ctypes.c_uint32),
+        ("prefix_len", ctypes.c_uint32),
+        ("status", ctypes.c_int32),
+        ("tasks", ctypes.c_uint32),
+        ("launches", ctypes.c_uint32),
+        ("max_batch", ctypes.c_uint32),
+        ("count", U128),
+        ("submit_s", ctypes.c_double),
+        ("start_s", ctypes.c_double),
+        ("finish_s", ctypes.c_double),
+        ("error", ctypes.c_char * 160),
+    ]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [
+        ("submitted", ctypes.c_uint64),
+        ("completed", ctypes.c_uint64),
+        ("failed", ctypes.c_uint64),
+        ("pending", ctypes.c_uint64),
+        ("launches", ctypes.c_uint64),
+        ("tasks", ctypes.c_uint64),
+        ("busy_s", ctypes.c_double),
+        ("uptime_s", ctypes.c_double),
     ]

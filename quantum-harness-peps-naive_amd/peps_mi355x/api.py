@@ -7,6 +7,8 @@ Reference functions mirrored (crate nqueens-peps-naive, src/lib.rs):
   known_count                         src/lib.rs:5752-5784
   peak_rss_bytes                      src/lib.rs:5786-5850
   wide_crt_plan / reconstruct_crt     src/lib.rs:3859-3886, 4134-4163
+plus the prefix count, the batched (prefix-sliced) form and a request server
+(Server), which the reference does not have.
 Errors surface as PepsError carrying the library's message (the reference
 returns Err(String) with the same texts).
 """
@@ -15,12 +17,18 @@ import dataclasses
 import os
 
 from .abi import (
+    PEPS_E_INVALID,
+    PEPS_E_PENDING,
+    PEPS_POLICY_FIFO,
+    PEPS_POLICY_SHORTEST_FIRST,
     PEPS_SYMMETRY_NONE,
     PEPS_SYMMETRY_TOP_ROW_ORBITS,
     U128,
     ContractionResult,
     KernelStat,
     Options,
+    RequestResult,
+    ServerStats,
 )
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -100,6 +108,15 @@ def load():
     L.peps_profile_kernels.argtypes = [c.c_uint32, c.c_uint32, c.c_char_p, c.c_size_t]
     L.peps_kernel_stats.argtypes = [c.c_uint32, P(KernelStat), c.c_uint32, P(c.c_uint32), c.c_char_p,
                                     c.c_size_t]
+    S = c.c_void_p
+    L.peps_server_create.argtypes = [P(Options), c.c_uint32, c.c_uint32, P(S), c.c_char_p, c.c_size_t]
+    L.peps_server_submit.argtypes = [S, c.c_uint32, P(c.c_uint32), c.c_uint32, P(c.c_uint64), c.c_char_p,
+                                     c.c_size_t]
+    L.peps_server_poll.argtypes = [S, c.c_uint64, P(RequestResult)]
+    L.peps_server_wait.argtypes = [S, c.c_uint64, c.c_double, P(RequestResult)]
+    L.peps_server_stats_get.argtypes = [S, P(ServerStats)]
+    L.peps_server_destroy.argtypes = [S]
+    L.peps_server_destroy.restype = None
     _lib = L
     return L
 
@@ -231,3 +248,94 @@ def kernel_stats(device: int = 0) -> dict:
     return {buf[i].name.decode(): {"launches": buf[i].launches, "seconds": buf[i].seconds,
                                    "algorithmic_bytes": buf[i].algorithmic_bytes}
             for i in range(min(cap, cnt.value))}
+
+
+@dataclasses.dataclass
+class RequestOutcome:
+    """One finished (or pending) server request."""
+    ticket: int
+    n: int
+    prefix_len: int
+    status: int
+    count: int
+    tasks: int
+    launches: int
+    max_batch: int
+    submit_s: float
+    start_s: float
+    finish_s: float
+    error: str
+
+    @property
+    def latency_s(self) -> float:
+        return self.finish_s - self.submit_s
+
+    @staticmethod
+    def of(r: "RequestResult") -> "RequestOutcome":
+        return RequestOutcome(r.ticket, r.n, r.prefix_len, r.status, r.count.value, r.tasks, r.launches,
+                              r.max_batch, r.submit_s, r.start_s, r.finish_s,
+                              r.error.decode(errors="replace"))
+
+
+class Server:
+    """Request server over the engine (peps_server_* in the C ABI).
+
+    ``submit(n, cols)`` queues the count of completions of the prefix ``cols``
+    (rows 0..k-1) and returns a ticket; ``wait(ticket)`` returns its
+    ``RequestOutcome``.  Requests of one N share launches (continuous
+    batching); a result does not depend on what else was in flight.
+    """
+
+    def __init__(self, options: PepsOptions = None, policy: str = "shortest", max_pending: int = 0):
+        pol = {"shortest": PEPS_POLICY_SHORTEST_FIRST, "fifo": PEPS_POLICY_FIFO}[policy]
+        self._h = ctypes.c_void_p()
+        e = _err()
+        ref, _keep = _opts(options)
+        _check(load().peps_server_create(ref, pol, max_pending, ctypes.byref(self._h), e, len(e)), e)
+
+    def submit(self, n: int, cols=()) -> int:
+        cols = list(cols)
+        arr = (ctypes.c_uint32 * max(1, len(cols)))(*cols)
+        t = ctypes.c_uint64()
+        e = _err()
+        _check(load().peps_server_submit(self._h, n, arr, len(cols), ctypes.byref(t), e, len(e)), e)
+        return t.value
+
+    def poll(self, ticket: int):
+        r = RequestResult()
+        rc = load().peps_server_poll(self._h, ticket, ctypes.byref(r))
+        if rc == PEPS_E_INVALID and r.ticket != ticket:
+            raise PepsError(rc, f"unknown ticket {ticket}")
+        return None if rc == PEPS_E_PENDING else RequestOutcome.of(r)
+
+    def wait(self, ticket: int, timeout: float = None) -> RequestOutcome:
+        r = RequestResult()
+        rc = load().peps_server_wait(self._h, ticket, -1.0 if timeout is None else float(timeout),
+                                     ctypes.byref(r))
+        if rc == PEPS_E_PENDING:
+            raise TimeoutError(f"request {ticket} not finished after {timeout} s")
+        if r.ticket != ticket:
+            raise PepsError(rc, f"unknown ticket {ticket}")
+        return RequestOutcome.of(r)
+
+    def stats(self) -> dict:
+        s = ServerStats()
+        load().peps_server_stats_get(self._h, ctypes.byref(s))
+        return {name: getattr(s, name) for name, _ in ServerStats._fields_}
+
+    def close(self) -> None:
+        if self._h:
+            load().peps_server_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

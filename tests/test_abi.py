@@ -27,7 +27,7 @@ def _has_gpu():
 
 def declared_symbols():
     text = open(HEADER).read()
-    return sorted(set(re.findall(r"^\s*(?:int|uint64_t|const char\*)\s+(peps_\w+)\s*\(", text, re.M)))
+    return sorted(set(re.findall(r"^\s*(?:int|uint64_t|void|const char\*)\s+(peps_\w+)\s*\(", text, re.M)))
 
 
 def test_library_exports_every_declared_symbol():
