@@ -122,7 +122,9 @@ typedef struct {
                                 the slice-id bits above the 3N-bit key allow) */
   uint32_t merge_passes;     /* LDS merge passes per final bucket (0: auto; 1..8) */
   uint32_t merge_waves;      /* merge grid in waves of resident workgroups (0: 4; 1..8) */
-  uint32_t reserved[2];
+  uint32_t site_counters;    /* sitewise tensor counters: 0 indexed (src/lib.rs:537-540),
+                                1 every C entry per partial-row visit (benchmarks/n14_layers.csv) */
+  uint32_t reserved[1];
 } peps_options;
 
 /* ---------------------------------------------------------------- solve -- */
@@ -208,6 +210,70 @@ int peps_profile_kernels(uint32_t device, uint32_t enable, char* err, size_t err
 /* Copy up to `cap` kernel stats; *count receives how many kernels have stats. */
 int peps_kernel_stats(uint32_t device, peps_kernel_stat* out, uint32_t cap, uint32_t* count,
                       char* err, size_t err_cap);
+
+/* ------------------------------------------------------ tensor fidelity -- */
+
+/* One entry of the explicit site tensor C: CEntry (src/lib.rs:88-92), legs
+ * in the VirtualLegs order (src/lib.rs:19-36), value as u64. */
+typedef struct {
+  uint8_t column_in, column_out, row_in, row_out;
+  uint8_t diag_dr_in, diag_dr_out, diag_dl_in, diag_dl_out;
+  uint64_t value;
+} peps_tensor_entry;
+
+enum { PEPS_ROW_OPERATOR_COMPILED = 0, PEPS_ROW_OPERATOR_SITEWISE = 1 };
+
+/* SiteTensorC::sec_vi() (src/lib.rs:102-139): the 17 entries in the
+ * reference's legs_key order.  Writes up to cap entries; *count = 17. */
+int peps_site_tensor_c(peps_tensor_entry* out, uint32_t cap, uint32_t* count);
+
+/* CompiledRowOperator::compile (src/lib.rs:425-485): fails closed
+ * (PEPS_E_INVALID with the reference's message) unless C is exactly the 16
+ * identity pass-throughs plus one unit all-channel 0 -> 1 entry; on success
+ * *occupied is that entry. */
+int peps_compile_row_operator(const peps_tensor_entry* c, uint32_t count, peps_tensor_entry* occupied,
+                              char* err, size_t err_cap);
+
+/* contract_rows_sitewise (src/lib.rs:5629-5631 -> contract_rows_with_backend,
+ * :5633-5750): every row applies the explicit C tensor literally, site by
+ * site, to a frontier held in HBM (no row-0 orbit slicing).  Layer tensor
+ * counters as the reference's sitewise backend (opts->site_counters selects
+ * the pre-index revision's count); row_operator_* are 0. */
+int peps_contract_rows_sitewise(uint32_t n, const peps_options* opts, peps_contraction_result* out,
+                                char* err, size_t err_cap);
+
+/* The same literal contraction with a caller-supplied C (up to 64 entries,
+ * binary legs).  Counts are exact modulo the product of the CRT primes
+ * (opts->crt_lanes, default the plan for N); no N! bound is applied, so a
+ * C other than Sec. VI's may be contracted. */
+int peps_contract_rows_with_tensor(uint32_t n, const peps_tensor_entry* c, uint32_t count,
+                                   const peps_options* opts, peps_contraction_result* out, char* err,
+                                   size_t err_cap);
+
+/* One row of a layer trace: the merged boundary layer after `row`. */
+typedef struct {
+  uint32_t row;           /* 0-based, as LayerMetric.row */
+  uint64_t support;       /* states in the layer */
+  peps_u128 weight_sum;   /* exact sum of the weights */
+  peps_u128 transitions;  /* successor terms that formed the layer (completed_row_terms) */
+  uint8_t sha256[32];     /* SHA-256 of the canonical layer */
+} peps_layer_digest;
+
+/* Layer trace certificate: runs the materialised contraction and hashes each
+ * merged layer in canonical form -- states sorted by (columns, diag_dr,
+ * diag_dl, weight), each serialised as 40 little-endian bytes: columns u64,
+ * diag_dr u64, diag_dl u64, weight u128 -- as proposed in
+ * docs/sec_vi_row_automaton_equivalence.md:196-217.  row_operator:
+ * PEPS_ROW_OPERATOR_COMPILED (C must compile; fails closed otherwise) or
+ * PEPS_ROW_OPERATOR_SITEWISE (literal C).  c == NULL: Sec. VI's C.
+ * symmetry: PEPS_SYMMETRY_* (orbit slicing only with the compiled operator).
+ * Writes up to cap digests; *rows = N. */
+int peps_trace_layers(uint32_t n, uint32_t row_operator, uint32_t symmetry, const peps_tensor_entry* c,
+                      uint32_t c_count, const peps_options* opts, peps_layer_digest* out, uint32_t cap,
+                      uint32_t* rows, char* err, size_t err_cap);
+
+/* SHA-256 (FIPS 180-4) of a byte string, as used by the layer trace. */
+int peps_sha256(const void* data, size_t len, uint8_t out[32]);
 
 /* --------------------------------------------------------------- serving -- */
 

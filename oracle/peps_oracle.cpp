@@ -118,6 +118,19 @@ SiteTensorC SiteTensorC::from_b(const std::vector<BEntry>& b) {
 
 SiteTensorC SiteTensorC::sec_vi() { return from_b(site_tensor_b()); }
 
+SiteTensorC SiteTensorC::from_entries(const std::vector<CEntry>& entries) {
+  SiteTensorC c;
+  c.entries = entries;
+  for (const CEntry& ce : c.entries) {
+    for (uint8_t v : {ce.legs.column_in, ce.legs.column_out, ce.legs.row_in, ce.legs.row_out,
+                      ce.legs.diag_dr_in, ce.legs.diag_dr_out, ce.legs.diag_dl_in, ce.legs.diag_dl_out})
+      if (v > 1) throw OracleError("C entry with a non-binary virtual leg");
+    c.by_input[input_key(ce.legs.column_in, ce.legs.row_in, ce.legs.diag_dr_in, ce.legs.diag_dl_in)]
+        .push_back(ce);
+  }
+  return c;
+}
+
 const std::vector<CEntry>& SiteTensorC::matching(uint8_t col, uint8_t row, uint8_t dr,
                                                  uint8_t dl) const {
   return by_input[input_key(col, row, dr, dl)];
@@ -335,8 +348,10 @@ void contract_rows(unsigned n, const Options& opt, peps_contraction_result* out)
     return;
   }
   const bool hash = opt.backend == Backend::Hash || opt.backend == Backend::Sitewise;
-  SiteTensorC tensor = SiteTensorC::sec_vi();
-  CompiledRowOperator op = CompiledRowOperator::compile(tensor);
+  if (opt.custom_tensor && opt.backend != Backend::Sitewise)
+    throw OracleError("a custom C tensor needs the sitewise backend");
+  SiteTensorC tensor = opt.custom_tensor ? *opt.custom_tensor : SiteTensorC::sec_vi();
+  CompiledRowOperator op = CompiledRowOperator::compile(SiteTensorC::sec_vi());
   const unsigned first_row = opt.has_start ? opt.start_row : 0;
   const bool orbits = opt.symmetry == Symmetry::TopRowOrbits && !opt.has_start;
 
@@ -433,6 +448,7 @@ void contract_rows(unsigned n, const Options& opt, peps_contraction_result* out)
       reduce_sorted(next, row);
     }
     boundary.swap(next);
+    if (opt.on_layer) opt.on_layer(row, boundary);
 
     u128 output_weight = 0;
     for (const Record& r : boundary)
@@ -944,5 +960,52 @@ int oracle_row_transfer(uint32_t n, uint64_t cols, uint64_t dr, uint64_t dl, uin
 }
 
 uint64_t oracle_peak_rss_bytes(void) { return oracle::peak_rss_bytes(); }
+
+// Sitewise contraction of a caller-supplied C (peps_tensor_entry layout);
+// layer(user, row, count, states) receives every merged layer as count
+// quintuples (columns, diag_dr, diag_dl, weight_lo, weight_hi) in no order.
+// c == NULL: Sec. VI's C.  backend: 0 sort-reduce (compiled), 2 sitewise.
+typedef void (*oracle_layer_fn)(void* user, uint32_t row, uint64_t count, const uint64_t* states);
+
+int oracle_contract_rows_tensor(uint32_t n, uint32_t symmetry, uint32_t backend, const peps_tensor_entry* c,
+                                uint32_t count, uint32_t linear_scan_counters, oracle_layer_fn layer,
+                                void* user, peps_contraction_result* out, char* err, size_t cap) {
+  try {
+    oracle::Options o;
+    o.symmetry = symmetry ? oracle::Symmetry::TopRowOrbits : oracle::Symmetry::None;
+    o.backend = backend == 2 ? oracle::Backend::Sitewise : oracle::Backend::SortReduce;
+    o.linear_scan_counters = linear_scan_counters != 0;
+    oracle::SiteTensorC custom;
+    if (c) {
+      std::vector<oracle::CEntry> entries;
+      for (uint32_t i = 0; i < count; ++i)
+        entries.push_back(oracle::CEntry{
+            oracle::Legs{c[i].column_in, c[i].column_out, c[i].row_in, c[i].row_out, c[i].diag_dr_in,
+                         c[i].diag_dr_out, c[i].diag_dl_in, c[i].diag_dl_out},
+            (oracle::u128)c[i].value});
+      custom = oracle::SiteTensorC::from_entries(entries);
+      o.custom_tensor = &custom;
+    }
+    if (layer) {
+      o.on_layer = [&](unsigned row, const std::vector<std::pair<oracle::u128, oracle::u128>>& recs) {
+        std::vector<uint64_t> flat;
+        flat.reserve(recs.size() * 5);
+        for (const auto& r : recs) {
+          oracle::Boundary b = oracle::unpack(r.first, n);
+          flat.push_back(b.columns);
+          flat.push_back(b.diag_dr);
+          flat.push_back(b.diag_dl);
+          flat.push_back((uint64_t)r.second);
+          flat.push_back((uint64_t)(r.second >> 64));
+        }
+        layer(user, row, recs.size(), flat.data());
+      };
+    }
+    oracle::contract_rows(n, o, out);
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, err, cap);
+  }
+}
 
 }  // extern "C"

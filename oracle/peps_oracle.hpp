@@ -10,6 +10,7 @@
 
 #include <array>
 #include <cstdint>
+#include <functional>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -50,6 +51,9 @@ struct SiteTensorC {
   std::array<std::vector<CEntry>, 16> by_input;  // input_key buckets
   static SiteTensorC from_b(const std::vector<BEntry>& b);
   static SiteTensorC sec_vi();
+  // A caller-supplied C (mutation tests): entries kept in the given order and
+  // bucketed by input_key like from_b.
+  static SiteTensorC from_entries(const std::vector<CEntry>& entries);
   const std::vector<CEntry>& matching(uint8_t col, uint8_t row, uint8_t dr, uint8_t dl) const;
 };
 
@@ -111,6 +115,10 @@ struct Options {
   Boundary start{0, 0, 0};
   unsigned start_row = 0;
   u128 start_weight = 1;
+  // Sitewise backend only: contract this C instead of Sec. VI's (no compile).
+  const SiteTensorC* custom_tensor = nullptr;
+  // Called after every row with the merged layer (packed key, weight).
+  std::function<void(unsigned row, const std::vector<std::pair<u128, u128>>& layer)> on_layer;
 };
 
 // contract_rows_sort_reduce_with_modes src/lib.rs:5287-5415,

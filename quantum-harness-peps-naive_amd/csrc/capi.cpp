@@ -19,68 +19,15 @@
 #include "host_math.h"
 
 using peps::u128;
+using peps::abi;
+using peps::fail;
+using peps::fill_result;
+using peps::rss_bytes;
 using peps::Resolved;
 using peps::check_board;
 using peps::run_engine;
 
 namespace {
-
-int fail(int code, const std::string& msg, char* err, size_t cap) {
-  if (err && cap) {
-    size_t n = std::min(cap - 1, msg.size());
-    std::memcpy(err, msg.data(), n);
-    err[n] = 0;
-  }
-  return code;
-}
-
-peps_u128 abi(u128 v) { return peps_u128{(uint64_t)v, (uint64_t)(v >> 64)}; }
-
-uint64_t rss_bytes() {
-  struct rusage ru;
-  if (getrusage(RUSAGE_SELF, &ru) != 0) return 0;
-  return (uint64_t)ru.ru_maxrss * 1024ull;
-}
-
-void fill_result(unsigned n, const peps::RunOut& ro, const std::vector<uint64_t>& primes,
-                 double elapsed, peps_contraction_result* out) {
-  out->n = n;
-  out->count = abi(ro.count);
-  out->elapsed_s = elapsed;
-  out->peak_states = ro.peak_states;
-  out->tensor_entries_examined = abi(17);
-  out->tensor_entries_matched = abi(17);
-  out->row_operator_candidates = abi(ro.total_candidates);
-  out->row_operator_matched = abi(ro.total_matched);
-  out->layer_count = (uint32_t)std::min<size_t>(ro.layers.size(), PEPS_MAX_N);
-  for (uint32_t i = 0; i < out->layer_count; ++i) {
-    const peps::LayerOut& l = ro.layers[i];
-    peps_layer_metric& m = out->layers[i];
-    m.row = l.row;
-    m.input_states = l.input_states;
-    m.tensor_entries_examined = abi(0);
-    m.tensor_entries_matched = abi(0);
-    m.row_operator_candidates = abi(l.candidates);
-    m.row_operator_matched = abi(l.matched);
-    m.completed_row_terms = abi(l.completed);
-    m.output_states = l.output_states;
-    m.output_weight = abi(l.output_weight);
-    m.elapsed_s = l.elapsed_s;
-    m.peak_rss_bytes = rss_bytes();
-    m.device_bytes = l.device_bytes;
-    m.partition_levels = l.levels;
-    m.final_buckets = l.buckets;
-  }
-  out->crt_lanes = (uint32_t)primes.size();
-  for (size_t l = 0; l < primes.size() && l < PEPS_MAX_LANES; ++l) {
-    out->primes[l] = primes[l];
-    out->residues[l] = l < ro.residues.size() ? ro.residues[l] : 0;
-  }
-  out->device_peak_bytes = ro.device_peak_bytes;
-  out->kernel_s = ro.kernel_s;
-  out->alloc_s = ro.alloc_s;
-  out->peak_rss_bytes = rss_bytes();
-}
 
 peps_progress_fn g_progress = nullptr;
 void* g_progress_user = nullptr;

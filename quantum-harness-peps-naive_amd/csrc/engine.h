@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <functional>
 #include <map>
 #include <mutex>
 #include <string>
@@ -41,6 +42,17 @@ struct RunSpec {
   uint64_t budget_bytes = 0;       // arena cap (0: 85% of free HBM at first use)
   bool per_slice = false;          // report a count per slice id
   bool use_graph = false;          // record the size plan; replay repeats as one HIP graph
+  // Materialised mode: every row's merged frontier is written to HBM (keys and
+  // residues) and expanded from there, through the compiled relation or, when
+  // `site` is set, the literal sitewise C tensor.
+  bool materialize = false;
+  const SiteTable* site = nullptr;
+  bool bound_check = true;         // per-row sums and the count must not exceed N!
+  // Materialised mode: called after every row with the merged layer
+  // (keys[S], residues[l * S + i]); a nonzero return aborts the run.
+  std::function<int(unsigned row, const std::vector<uint64_t>& keys, const std::vector<uint32_t>& res,
+                    std::string* err)>
+      on_layer;
 };
 
 struct LayerOut {
@@ -55,6 +67,8 @@ struct LayerOut {
   uint64_t device_bytes;
   uint32_t levels;
   uint32_t buckets;
+  u128 site_visits = 0;   // literal sitewise rows: partial-row visits
+  u128 site_matched = 0;  // literal sitewise rows: C entries applied
 };
 
 struct RunOut {
@@ -126,6 +140,7 @@ class Engine {
   hipStream_t stream_ = nullptr;
   hipEvent_t ev_start_ = nullptr, ev_stop_ = nullptr;
   int pipeline(const RunSpec& spec, Plan* plan, bool eager, RunOut* out, std::string* err);
+  int materialized(const RunSpec& spec, RunOut* out, std::string* err);
   int finish(const RunSpec& spec, const Plan& plan, RunOut* out, std::string* err);
   std::map<std::string, Plan> plans_;
   uint64_t generation_ = 1;        // bumped whenever a device buffer is reallocated
@@ -149,6 +164,8 @@ class Engine {
   size_t counters_rows_cap_ = 0;
   uint64_t* seed_ = nullptr;           // seed frontier (SoA)
   size_t seed_cap_ = 0;
+  uint64_t* front_ = nullptr;          // materialised frontier (SoA), u64 units
+  size_t front_cap_ = 0;
   uint64_t budget_ = 0;
   Legs occupied_{};
   uint32_t cu_count_ = 256;

@@ -294,6 +294,12 @@ int Engine::run(const RunSpec& spec, RunOut* out, std::string* err) {
   }
   out->replayed = false;
   int rc;
+  if (spec.materialize || spec.site) {
+    rc = materialized(spec, out, err);
+    if (rc) return rc;
+    out->elapsed_s = std::chrono::duration<double>(Clock::now() - t0).count();
+    return 0;
+  }
   if (spec.use_graph) {
     const std::string key = plan_key(spec);
     auto it = plans_.find(key);
@@ -824,6 +830,271 @@ int Engine::finish(const RunSpec& spec, const Plan& plan, RunOut* out, std::stri
       }
     }
   }
+  return 0;
+}
+
+// Materialised row loop: the merged frontier of every row is written to HBM
+// (front_, SoA) and expanded from there.  Used for the literal sitewise C
+// transfer (contract_rows_sitewise, src/lib.rs:5629-5750) and for layer
+// traces; each row synchronises with the host.
+int Engine::materialized(const RunSpec& spec, RunOut* out, std::string* err) {
+  const unsigned n = spec.n;
+  const int L = (int)spec.primes.size();
+  const size_t rb = record_bytes(L);
+  const size_t fb = 8 + 4 * (size_t)L;  // frontier bytes per state
+  const double R = 8.0 + 4.0 * L;
+  const bool sitewise = spec.site != nullptr;
+  if (sitewise && spec.orbit_first_row) {
+    *err = "the literal sitewise row transfer runs without row-0 orbit slicing";
+    return 1;
+  }
+  if (spec.per_slice) {
+    *err = "materialised rows do not report per-slice counts";
+    return 1;
+  }
+  Primes pr{};
+  for (int l = 0; l < L; ++l) pr.p[l] = (uint32_t)spec.primes[l];
+  u128 fact;
+  factorial(n, &fact);
+
+  RowGeom g0{};
+  g0.n = n;
+  g0.board_mask = (uint32_t)mask_bits(n);
+  g0.state_mask = mask_bits(3 * n);
+  g0.orbit_mask = g0.board_mask;
+  g0.col_in = occupied_.column_in ? g0.board_mask : 0u;
+  g0.col_out = occupied_.column_out ? g0.board_mask : 0u;
+  g0.dr_in = occupied_.diag_dr_in ? g0.board_mask : 0u;
+  g0.dr_out = occupied_.diag_dr_out ? g0.board_mask : 0u;
+  g0.dl_in = occupied_.diag_dl_in ? g0.board_mask : 0u;
+  g0.dl_out = occupied_.diag_dl_out ? g0.board_mask : 0u;
+  const uint64_t final_mask = g0.board_mask;
+
+  const uint32_t rbits = std::max<uint32_t>(1, std::min<uint32_t>(spec.radix_bits, kMaxDigitBits));
+  const uint32_t table_slots = spec.tile_states <= 1024 ? 2048u : 4096u;
+  const uint64_t target = (uint64_t)std::max<uint32_t>(256, std::min<uint32_t>(spec.tile_states, merge_cap(table_slots)));
+  const uint64_t merge_grid = (uint64_t)cu_count_ * merge_resident_blocks(L, table_slots) *
+                              std::max<uint32_t>(1, std::min<uint32_t>(spec.merge_waves, 8));
+
+  const size_t ncount = (size_t)(n + 1) * kNumCounters;
+  int rc = ensure_buffer(&counters_rows_, &counters_rows_cap_, ncount, err);
+  if (rc) return rc;
+  uint64_t* spare = counters_rows_ + (size_t)n * kNumCounters;
+
+  // Seed frontier, stride S0.
+  const uint64_t S0 = spec.seeds.size();
+  rc = ensure_buffer(&front_, &front_cap_, (S0 * fb + 7) / 8, err);
+  if (rc) return rc;
+  {
+    std::vector<uint64_t> host((S0 * fb + 7) / 8, 0);
+    uint32_t* res = (uint32_t*)(host.data() + S0);
+    for (uint64_t i = 0; i < S0; ++i) {
+      host[i] = spec.seeds[i].key;
+      for (int l = 0; l < L; ++l) res[(size_t)l * S0 + i] = (uint32_t)(spec.seeds[i].weight % spec.primes[l]);
+    }
+    HIP_TRY(hipStreamSynchronize(stream_));
+    HIP_TRY(hipMemcpy(front_, host.data(), S0 * fb, hipMemcpyHostToDevice));
+  }
+  out->layers.clear();
+  out->peak_states = S0;
+  out->total_candidates = out->total_matched = 0;
+  out->device_peak_bytes = 0;
+  out->residues.assign(L, 0);
+  HIP_TRY(hipEventRecord(ev_start_, stream_));
+  LAUNCH("fill", 8.0 * ncount, launch_fill(counters_rows_, ncount, 0, stream_));
+
+  std::vector<uint64_t> h(kNumCounters);
+  auto read_counters = [&](const uint64_t* dev) -> int {
+    HIP_TRY(hipMemcpyAsync(h_counters_, dev, sizeof(uint64_t) * kNumCounters, hipMemcpyDeviceToHost, stream_));
+    SYNC_AND_RESOLVE();
+    std::copy(h_counters_, h_counters_ + kNumCounters, h.begin());
+    return 0;
+  };
+
+  uint64_t S = S0, stride = S0;
+  std::vector<uint64_t> final_res(L, 0);
+  for (unsigned row = spec.first_row; row < n; ++row) {
+    const auto rt0 = Clock::now();
+    LayerOut lo{};
+    lo.row = row;
+    lo.input_states = S;
+    uint64_t* cr = counters_rows_ + (size_t)row * kNumCounters;
+    const bool orbit = !sitewise && spec.orbit_first_row && row == 0;
+    RowGeom g = g0;
+    const uint64_t* keys = front_;
+    const uint32_t* res = (const uint32_t*)(front_ + stride);
+    uint64_t C = 0;
+    if (S) {
+      if (sitewise) {
+        LAUNCH("site_count", 8.0 * S, launch_site_count(keys, S, g, *spec.site, cr, stream_));
+      } else {
+        if (orbit) {
+          LAUNCH("count", 8.0 * S, launch_count(keys, S, g, cr + kMatchedSlot, stream_));
+          g.orbit_row = 1;
+          g.orbit_mask = (uint32_t)mask_bits((n - 1) / 2 + 1);
+          g.centre_bit = (n % 2) ? (1u << (n / 2)) : 0u;
+        }
+        LAUNCH("count", 8.0 * S, launch_count(keys, S, g, cr, stream_));
+      }
+      rc = read_counters(cr);
+      if (rc) return rc;
+      if (h[kError] & 8) {
+        *err = "a parent needs more than " + std::to_string(kSitePartials) +
+               " partial rows in the sitewise C contraction of row " + std::to_string(row + 1);
+        return 1;
+      }
+      C = h[kCandidates];
+      if (sitewise) {
+        lo.site_visits = h[kSiteVisits];
+        lo.site_matched = h[kSiteMatched];
+      } else {
+        lo.candidates = (u128)n * S;
+        lo.matched = orbit ? (u128)h[kMatchedSlot] : (u128)C;
+      }
+    }
+    lo.completed = C;
+    uint64_t S_out = 0;
+    if (C) {
+      const unsigned B = C <= target ? 0u : ceil_log2((C + target - 1) / target);
+      const unsigned d1 = std::min<unsigned>(B, kFineBits);
+      std::vector<unsigned> pd;
+      for (unsigned used = d1; used < B;) {
+        const unsigned dk = std::min<unsigned>(rbits, B - used);
+        pd.push_back(dk);
+        used += dk;
+      }
+      const uint64_t per_unit = sitewise ? 1024 : 4096;
+      uint64_t tiles = std::min<uint64_t>(sitewise ? 2048 : 1024, std::max<uint64_t>(1, (S + per_unit - 1) / per_unit));
+      const uint64_t per_tile = (S + tiles - 1) / tiles;
+      tiles = (S + per_tile - 1) / per_tile;
+      const uint32_t D1 = 1u << d1;
+      rc = ensure_region(0, C * rb, err);
+      if (!rc) rc = ensure_buffer(&hist_, &hist_cap_, tiles * D1, err);
+      if (!rc) rc = ensure_buffer(&offsets_, &offsets_cap_, tiles * D1, err);
+      if (!rc) rc = ensure_buffer(&partial_, &partial_cap_, (tiles * D1) / 4096 + 2, err);
+      if (!rc) rc = ensure_buffer(&segs_[0], &segs_cap_[0], (size_t)D1 + 1, err);
+      if (rc) return rc;
+      if (sitewise) {
+        LAUNCH("site_hist", 8.0 * S,
+               launch_site_hist(keys, S, g, *spec.site, d1, (uint32_t)tiles, per_tile, hist_, stream_));
+      } else {
+        LAUNCH("expand_hist", 8.0 * S,
+               launch_expand_hist(keys, S, g, d1, (uint32_t)tiles, per_tile, hist_, stream_));
+      }
+      LAUNCH("scan", 12.0 * tiles * D1,
+             launch_scan(hist_, D1, (uint32_t)tiles, 1, offsets_, segs_[0], partial_, cr, stream_));
+      if (sitewise) {
+        LAUNCH("site_scatter", R * (S + C),
+               launch_site_scatter(L, keys, res, S, stride, g, *spec.site, pr, d1, (uint32_t)tiles, per_tile,
+                                   offsets_, region_[0], region_bytes_[0] / rb, spare, stream_));
+      } else {
+        LAUNCH("expand_scatter", R * (S + C),
+               launch_expand_scatter(L, keys, res, S, stride, g, pr, d1, (uint32_t)tiles, per_tile, offsets_,
+                                     region_[0], region_bytes_[0] / rb, spare, stream_));
+      }
+      int cur = 0, sidx = 0;
+      uint64_t nseg = (uint64_t)D1;
+      unsigned used = d1;
+      for (unsigned dk : pd) {
+        used += dk;
+        const int to = cur ^ 1;
+        rc = ensure_region(to, C * rb, err);
+        if (!rc) rc = ensure_buffer(&segs_[sidx ^ 1], &segs_cap_[sidx ^ 1], (size_t)(nseg << dk) + 1, err);
+        if (rc) return rc;
+        LAUNCH("partition", 2.0 * R * C,
+               launch_partition(L, region_[cur], region_[to], segs_[sidx], (uint32_t)nseg, 64 - used, dk,
+                                segs_[sidx ^ 1], region_bytes_[to] / rb, spare, stream_));
+        cur = to;
+        sidx ^= 1;
+        nseg <<= dk;
+      }
+      lo.levels = (d1 ? 1 : 0) + (uint32_t)pd.size();
+      lo.buckets = (uint32_t)nseg;
+      // The input frontier has been consumed: the merged states overwrite it.
+      rc = ensure_buffer(&front_, &front_cap_, (C * fb + 7) / 8, err);
+      if (rc) return rc;
+      const uint32_t W = (uint32_t)std::min<uint64_t>(nseg, merge_grid);
+      LAUNCH("merge_emit", R * 2.0 * C,
+             launch_merge_expand(kMergeEmit, L, region_[cur], segs_[sidx], (uint32_t)nseg, W, g0, pr, 0, nullptr,
+                                 nullptr, front_, C, final_mask, nullptr, 0, cr, table_slots, stream_));
+      rc = read_counters(cr);
+      if (rc) return rc;
+      if (h[kError]) {
+        *err = (h[kError] & 1) ? "LDS merge table overflow in row " + std::to_string(row + 1)
+                               : "frontier capacity exceeded in row " + std::to_string(row + 1);
+        return 4;
+      }
+      S_out = h[kOutStates];
+      if (h[kEmitted] != S_out) {
+        *err = "merged state count disagrees with the states written in row " + std::to_string(row + 1);
+        return 4;
+      }
+      std::vector<uint64_t> w(L);
+      for (int l = 0; l < L; ++l) {
+        w[l] = h[kWeight0 + l] % spec.primes[l];
+        final_res[l] = h[kFinal0 + l] % spec.primes[l];
+      }
+      std::string cerr;
+      if (!crt_reconstruct(w.data(), spec.primes.data(), L, &lo.output_weight, &cerr)) {
+        *err = cerr;
+        return 2;
+      }
+      if (spec.bound_check && lo.output_weight > fact) {
+        *err = "CRT reconstruction exceeds certified N! count bound after row " + std::to_string(row + 1);
+        return 2;
+      }
+      stride = C;
+    } else {
+      std::fill(final_res.begin(), final_res.end(), 0);
+    }
+    lo.output_states = S_out;
+    if (spec.on_layer) {
+      std::vector<uint64_t> hk(S_out);
+      std::vector<uint32_t> hr((size_t)L * S_out);
+      if (S_out) {
+        HIP_TRY(hipMemcpy(hk.data(), front_, S_out * 8, hipMemcpyDeviceToHost));
+        const uint32_t* dres = (const uint32_t*)(front_ + stride);
+        for (int l = 0; l < L; ++l)
+          HIP_TRY(hipMemcpy(hr.data() + (size_t)l * S_out, dres + (size_t)l * stride, S_out * 4,
+                            hipMemcpyDeviceToHost));
+      }
+      rc = spec.on_layer(row, hk, hr, err);
+      if (rc) return rc;
+    }
+    lo.device_bytes = arena_bytes() + front_cap_ * 8 + hist_cap_ * 4 + offsets_cap_ * 8;
+    lo.elapsed_s = std::chrono::duration<double>(Clock::now() - rt0).count();
+    out->layers.push_back(lo);
+    out->total_candidates += lo.candidates;
+    out->total_matched += lo.matched;
+    out->peak_states = std::max<uint64_t>(out->peak_states, S_out);
+    out->device_peak_bytes = std::max<uint64_t>(out->device_peak_bytes, lo.device_bytes);
+    S = S_out;
+  }
+  HIP_TRY(hipEventRecord(ev_stop_, stream_));
+  HIP_TRY(hipEventSynchronize(ev_stop_));
+  float ms = 0;
+  HIP_TRY(hipEventElapsedTime(&ms, ev_start_, ev_stop_));
+  out->kernel_s = ms / 1000.0;
+  std::vector<uint64_t> sp(kNumCounters);
+  HIP_TRY(hipMemcpy(sp.data(), spare, sizeof(uint64_t) * kNumCounters, hipMemcpyDeviceToHost));
+  if (sp[kError]) {
+    *err = "candidate buffer capacity exceeded";
+    return 4;
+  }
+  u128 count = 0;
+  if (S) {
+    std::string cerr;
+    if (!crt_reconstruct(final_res.data(), spec.primes.data(), L, &count, &cerr)) {
+      *err = cerr;
+      return 2;
+    }
+    if (spec.bound_check && count > fact) {
+      *err = "CRT reconstruction exceeds certified N! count bound";
+      return 2;
+    }
+  }
+  out->count = count;
+  for (int l = 0; l < L; ++l) out->residues[l] = S ? final_res[l] : 0;
   return 0;
 }
 

@@ -1,9 +1,75 @@
 // frontend.cpp -- option resolution, prefix tasks and sliced launches.
 #include "frontend.h"
 
+#include <sys/resource.h>
+
 #include <algorithm>
+#include <cstring>
 
 namespace peps {
+
+int fail(int code, const std::string& msg, char* err, size_t cap) {
+  if (err && cap) {
+    const size_t n = std::min(cap - 1, msg.size());
+    std::memcpy(err, msg.data(), n);
+    err[n] = 0;
+  }
+  return code;
+}
+
+uint64_t rss_bytes() {
+  struct rusage ru;
+  if (getrusage(RUSAGE_SELF, &ru) != 0) return 0;
+  return (uint64_t)ru.ru_maxrss * 1024ull;
+}
+
+void fill_result(unsigned n, const RunOut& ro, const std::vector<uint64_t>& primes, double elapsed,
+                 peps_contraction_result* out, TensorCounters tc, uint64_t c_entries) {
+  out->n = n;
+  out->count = abi(ro.count);
+  out->elapsed_s = elapsed;
+  out->peak_states = ro.peak_states;
+  u128 examined = tc == TensorCounters::kCompiled ? 17 : 0, matched = examined;
+  out->row_operator_candidates = abi(ro.total_candidates);
+  out->row_operator_matched = abi(ro.total_matched);
+  out->layer_count = (uint32_t)std::min<size_t>(ro.layers.size(), PEPS_MAX_N);
+  for (uint32_t i = 0; i < out->layer_count; ++i) {
+    const LayerOut& l = ro.layers[i];
+    peps_layer_metric& m = out->layers[i];
+    m.row = l.row;
+    m.input_states = l.input_states;
+    u128 le = 0, lm = 0;
+    if (tc != TensorCounters::kCompiled) {
+      lm = l.site_matched;
+      le = tc == TensorCounters::kSitewiseLinear ? l.site_visits * (u128)c_entries : l.site_matched;
+    }
+    examined += le;
+    matched += lm;
+    m.tensor_entries_examined = abi(le);
+    m.tensor_entries_matched = abi(lm);
+    m.row_operator_candidates = abi(l.candidates);
+    m.row_operator_matched = abi(l.matched);
+    m.completed_row_terms = abi(l.completed);
+    m.output_states = l.output_states;
+    m.output_weight = abi(l.output_weight);
+    m.elapsed_s = l.elapsed_s;
+    m.peak_rss_bytes = rss_bytes();
+    m.device_bytes = l.device_bytes;
+    m.partition_levels = l.levels;
+    m.final_buckets = l.buckets;
+  }
+  out->tensor_entries_examined = abi(examined);
+  out->tensor_entries_matched = abi(matched);
+  out->crt_lanes = (uint32_t)primes.size();
+  for (size_t l = 0; l < primes.size() && l < PEPS_MAX_LANES; ++l) {
+    out->primes[l] = primes[l];
+    out->residues[l] = l < ro.residues.size() ? ro.residues[l] : 0;
+  }
+  out->device_peak_bytes = ro.device_peak_bytes;
+  out->kernel_s = ro.kernel_s;
+  out->alloc_s = ro.alloc_s;
+  out->peak_rss_bytes = rss_bytes();
+}
 
 int resolve_options(unsigned n, const peps_options* o, Resolved* r, std::string* err) {
   u128 prod, fact;
@@ -59,6 +125,11 @@ int resolve_options(unsigned n, const peps_options* o, Resolved* r, std::string*
     }
     r->merge_waves = o->merge_waves;
   }
+  if (o->site_counters > 1) {
+    *err = "site_counters must be 0 (indexed) or 1 (linear scan)";
+    return PEPS_E_INVALID;
+  }
+  r->site_counters = o->site_counters;
   return PEPS_OK;
 }
 

@@ -17,6 +17,24 @@
 
 namespace peps {
 
+// Copies `msg` into the caller's error buffer and returns `code`.
+int fail(int code, const std::string& msg, char* err, size_t cap);
+inline peps_u128 abi(u128 v) { return peps_u128{(uint64_t)v, (uint64_t)(v >> 64)}; }
+// peak_rss_bytes (src/lib.rs:5786-5850): the process's maximum resident set.
+uint64_t rss_bytes();
+
+// How the tensor counters of a result are reported.
+enum class TensorCounters {
+  kCompiled,        // 17 / 17 for the compiled operator, 0 per layer (src/lib.rs:5323-5324)
+  kSitewise,        // per layer: entries examined through the input index = applied entries
+  kSitewiseLinear,  // per layer: every C entry scanned per partial-row visit (pre-index revision)
+};
+
+// ContractionResult (src/lib.rs:403-415) from an engine run.
+void fill_result(unsigned n, const RunOut& ro, const std::vector<uint64_t>& primes, double elapsed,
+                 peps_contraction_result* out, TensorCounters tc = TensorCounters::kCompiled,
+                 uint64_t c_entries = 17);
+
 // peps_options after validation and defaults.
 struct Resolved {
   std::vector<uint64_t> primes;
@@ -29,6 +47,7 @@ struct Resolved {
   uint32_t slice_group = 0;
   uint32_t merge_passes = 1;
   uint32_t merge_waves = 4;
+  uint32_t site_counters = 0;
 };
 
 // Lane count: the reference plan (src/lib.rs:3859-3886) unless the caller

@@ -62,7 +62,9 @@ bool compile_row_operator(const std::vector<TensorEntry>& c, Legs* occupied, std
     }
     if (pass) {
       if (seen[in_key(l)]) {
-        *err = "duplicate C pass-through signature";
+        const unsigned k = in_key(l);
+        *err = "duplicate C pass-through signature ";
+        for (int b = 3; b >= 0; --b) err->push_back((char)('0' + ((k >> b) & 1u)));
         return false;
       }
       seen[in_key(l)] = true;

@@ -51,10 +51,28 @@ enum Counter : int {
   kFinal0 = 6,       // .. kFinal0 + L - 1: same restricted to columns == board mask
   kError = 10,       // nonzero: an LDS merge table overflowed
   kMatchedSlot = 11, // legal positions before the row-0 orbit filter
-  kNumCounters = 12
+  kSiteVisits = 12,  // sitewise: partial-row visits (one per partial per site)
+  kSiteMatched = 13, // sitewise: C entries applied (= entries examined through the input index)
+  kEmitted = 14,     // EMIT merge: merged states written to the frontier
+  kNumCounters = 15
 };
 
-enum MergeMode : int { kMergeCount = 0, kMergeScatter = 1, kMergeLast = 2 };
+// COUNT / SCATTER expand the merged states into the next row; LAST only sums;
+// EMIT writes the merged states out as the next frontier (materialised mode).
+enum MergeMode : int { kMergeCount = 0, kMergeScatter = 1, kMergeLast = 2, kMergeEmit = 3 };
+
+// The explicit C tensor as the literal sitewise kernels read it.  Entries are
+// grouped by input index col | row << 1 | dr << 2 | dl << 3 (the reference's
+// input_key, src/lib.rs:157-162); entry e's output legs are packed the same
+// way in out[e] and its value is held modulo every CRT prime.
+constexpr int kSiteMaxEntries = 64;
+constexpr int kSitePartials = 24;   // partial rows per parent (N + 1 for the Sec. VI tensor)
+struct SiteTable {
+  uint8_t first[16];
+  uint8_t count[16];
+  uint8_t out[kSiteMaxEntries];
+  uint32_t value[kSiteMaxEntries][kMaxLanes];
+};
 
 size_t record_bytes(int lanes);
 
@@ -72,6 +90,23 @@ hipError_t launch_expand_scatter(int lanes, const uint64_t* keys, const uint32_t
                                  uint32_t digit_bits, uint32_t tiles, uint64_t per_tile,
                                  const uint64_t* offsets, void* out, uint64_t cap,
                                  uint64_t* counters, hipStream_t stream);
+
+// Literal sitewise row transfer (contract_one_row_sitewise, src/lib.rs:513-592)
+// of a materialised frontier (keys[S], residues res[l * res_stride + i]):
+// COUNT adds successors, partial-row visits and applied entries to counters
+// (kCandidates, kSiteVisits, kSiteMatched; bit 3 of kError if a parent needs
+// more than kSitePartials partial rows); HIST and SCATTER have the shapes of
+// launch_expand_hist / launch_expand_scatter.
+hipError_t launch_site_count(const uint64_t* keys, uint64_t S, const RowGeom& g, const SiteTable& t,
+                             uint64_t* counters, hipStream_t stream);
+hipError_t launch_site_hist(const uint64_t* keys, uint64_t S, const RowGeom& g, const SiteTable& t,
+                            uint32_t digit_bits, uint32_t tiles, uint64_t per_tile, uint32_t* hist,
+                            hipStream_t stream);
+hipError_t launch_site_scatter(int lanes, const uint64_t* keys, const uint32_t* res, uint64_t S,
+                               uint64_t res_stride, const RowGeom& g, const SiteTable& t, const Primes& pr,
+                               uint32_t digit_bits, uint32_t tiles, uint64_t per_tile,
+                               const uint64_t* offsets, void* out, uint64_t cap, uint64_t* counters,
+                               hipStream_t stream);
 
 // p[0..n) = v (a kernel, so it can be captured into a HIP graph).
 hipError_t launch_fill(uint64_t* p, uint64_t n, uint64_t v, hipStream_t stream);
@@ -91,7 +126,9 @@ hipError_t launch_partition(int lanes, const void* in, void* out, const uint64_t
 // Fused merge of row r's buckets with the expansion into row r+1.
 // COUNT: hist[(1<<kFineBits)][W] of successor digits.  SCATTER: successors
 // written to out (capacity `cap` records) at offsets[(1<<digit_bits)][W].
-// LAST: no expansion.  Merged states with (key & final_mask) == final_mask
+// LAST: no expansion.  EMIT: the merged states are written to `out` as a
+// frontier (keys[cap] then residues[l * cap + i]), positions counted in
+// counters[kEmitted].  Merged states with (key & final_mask) == final_mask
 // (all columns set) are summed into kFinal0 and, if slice_sums is given,
 // into slice_sums[slice * L + l] (slice = key bits above 3N).
 // table_slots selects the LDS table (2048 or 4096 slots); a bucket of up to

@@ -338,6 +338,187 @@ __global__ __launch_bounds__(kBlock) void k_expand_scatter(
   if (staged) stage_flush<L, kExpandDigits>(st, staged, D, out, cap, counters);
 }
 
+// ---------------------------------------------------- literal sitewise C --
+//
+// contract_one_row_sitewise (src/lib.rs:513-592) applied to one parent per
+// thread: the partial rows of the parent live in LDS (two generations,
+// lane-interleaved so a wave's accesses hit distinct banks); at every site
+// each partial row looks up the C entries matching (column, row signal,
+// diag_dr, diag_dl) and forks once per entry, writing the entry's column
+// output at the site, its diag_dr output one site to the right and its
+// diag_dl output one site to the left (signals leaving the board are dropped:
+// v2).  The left boundary is v0 (signal 0); after the last site only partial
+// rows with signal 1 survive (v1).  Weights are multiplied by the entry value
+// modulo every CRT prime.
+
+constexpr int kSiteBlock = 64;
+
+__device__ __forceinline__ uint32_t mulmod(uint32_t a, uint32_t b, uint32_t p) {
+  return (uint32_t)(((uint64_t)a * b) % p);
+}
+
+template <int L, bool WEIGHTS, typename Emit>
+__device__ __forceinline__ bool site_walk(uint64_t parent, const uint32_t* w_in, const RowGeom& g,
+                                          const SiteTable& t, const Primes& pr, uint64_t* pk,
+                                          uint32_t* pw, uint64_t& visits, uint64_t& matched,
+                                          Emit&& emit) {
+  constexpr uint64_t kSig = 1ull << 63;
+  const int tid = threadIdx.x;
+  auto K = [&](int gen, int j) -> uint64_t& { return pk[(gen * kSitePartials + j) * kSiteBlock + tid]; };
+  auto Wt = [&](int gen, int j, int l) -> uint32_t& {
+    return pw[((gen * kSitePartials + j) * L + l) * kSiteBlock + tid];
+  };
+  const unsigned n = g.n;
+  int np = 1, cur = 0;
+  bool ok = true;
+  K(0, 0) = 0;
+  if (WEIGHTS) {
+#pragma unroll
+    for (int l = 0; l < L; ++l) Wt(0, 0, l) = w_in[l];
+  }
+  for (unsigned c = 0; c < n; ++c) {
+    const uint32_t col_in = (uint32_t)(parent >> c) & 1u;
+    const uint32_t dr_in = (uint32_t)(parent >> (n + c)) & 1u;
+    const uint32_t dl_in = (uint32_t)(parent >> (2 * n + c)) & 1u;
+    const int nxt = cur ^ 1;
+    int nn = 0;
+    for (int j = 0; j < np; ++j) {
+      const uint64_t k = K(cur, j);
+      const uint32_t row_in = (uint32_t)(k >> 63);
+      const uint32_t idx = col_in | (row_in << 1) | (dr_in << 2) | (dl_in << 3);
+      const uint32_t e0 = t.first[idx], ec = t.count[idx];
+      visits += 1;
+      for (uint32_t e = e0; e < e0 + ec; ++e) {
+        matched += 1;
+        const uint32_t o = t.out[e];
+        uint64_t sk = k & ~kSig;
+        sk = (sk & ~(1ull << c)) | ((uint64_t)(o & 1u) << c);
+        if (c + 1 < n) {
+          const unsigned b = n + c + 1;
+          sk = (sk & ~(1ull << b)) | ((uint64_t)((o >> 2) & 1u) << b);
+        }
+        if (c > 0) {
+          const unsigned b = 2 * n + c - 1;
+          sk = (sk & ~(1ull << b)) | ((uint64_t)((o >> 3) & 1u) << b);
+        }
+        sk |= (uint64_t)((o >> 1) & 1u) << 63;
+        if (nn < kSitePartials) {
+          K(nxt, nn) = sk;
+          if (WEIGHTS) {
+#pragma unroll
+            for (int l = 0; l < L; ++l) Wt(nxt, nn, l) = mulmod(Wt(cur, j, l), t.value[e][l], pr.p[l]);
+          }
+          ++nn;
+        } else {
+          ok = false;
+        }
+      }
+    }
+    np = nn;
+    cur = nxt;
+  }
+  const uint64_t slice = parent & ~g.state_mask;
+  for (int j = 0; j < np; ++j) {
+    const uint64_t k = K(cur, j);
+    if (k >> 63) {
+      uint32_t w[L];
+#pragma unroll
+      for (int l = 0; l < L; ++l) w[l] = WEIGHTS ? Wt(cur, j, l) : 0u;
+      emit(slice | (k & ~kSig), w);
+    }
+  }
+  return ok;
+}
+
+__device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  return v;
+}
+
+__global__ __launch_bounds__(kSiteBlock) void k_site_count(const uint64_t* __restrict__ keys, uint64_t S,
+                                                           RowGeom g, SiteTable t,
+                                                           uint64_t* __restrict__ counters) {
+  __shared__ uint64_t pk[2 * kSitePartials * kSiteBlock];
+  uint64_t emitted = 0, visits = 0, matched = 0;
+  bool ok = true;
+  Primes none{};
+  for (uint64_t i = (uint64_t)blockIdx.x * kSiteBlock + threadIdx.x; i < S;
+       i += (uint64_t)gridDim.x * kSiteBlock)
+    ok &= site_walk<1, false>(keys[i], nullptr, g, t, none, pk, nullptr, visits, matched,
+                              [&](uint64_t, const uint32_t*) { emitted += 1; });
+  emitted = wave_sum_u64(emitted);
+  visits = wave_sum_u64(visits);
+  matched = wave_sum_u64(matched);
+  if (threadIdx.x == 0) {
+    if (emitted) atomicAdd((unsigned long long*)&counters[kCandidates], (unsigned long long)emitted);
+    if (visits) atomicAdd((unsigned long long*)&counters[kSiteVisits], (unsigned long long)visits);
+    if (matched) atomicAdd((unsigned long long*)&counters[kSiteMatched], (unsigned long long)matched);
+  }
+  if (!ok) atomicOr((unsigned long long*)&counters[kError], 8ull);
+}
+
+__global__ __launch_bounds__(kSiteBlock) void k_site_hist(const uint64_t* __restrict__ keys, uint64_t S,
+                                                          RowGeom g, SiteTable t, uint32_t digit_bits,
+                                                          uint64_t per_tile, uint32_t* __restrict__ hist) {
+  __shared__ uint64_t pk[2 * kSitePartials * kSiteBlock];
+  __shared__ uint32_t lh[kExpandDigits];
+  const uint32_t D = 1u << digit_bits;
+  for (uint32_t i = threadIdx.x; i < D; i += kSiteBlock) lh[i] = 0;
+  __syncthreads();
+  uint64_t visits = 0, matched = 0;
+  Primes none{};
+  const uint64_t lo = (uint64_t)blockIdx.x * per_tile;
+  const uint64_t hi = lo + per_tile < S ? lo + per_tile : S;
+  for (uint64_t i = lo + threadIdx.x; i < hi; i += kSiteBlock)
+    site_walk<1, false>(keys[i], nullptr, g, t, none, pk, nullptr, visits, matched,
+                        [&](uint64_t sk, const uint32_t*) { atomicAdd(&lh[top_digit(sk, digit_bits)], 1u); });
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < D; i += kSiteBlock) hist[(uint64_t)i * gridDim.x + blockIdx.x] = lh[i];
+}
+
+template <int L>
+__global__ __launch_bounds__(kSiteBlock) void k_site_scatter(
+    const uint64_t* __restrict__ keys, const uint32_t* __restrict__ res, uint64_t S, uint64_t res_stride,
+    RowGeom g, SiteTable t, Primes pr, uint32_t digit_bits, uint64_t per_tile,
+    const uint64_t* __restrict__ offsets, Rec<L>* __restrict__ out, uint64_t cap,
+    uint64_t* __restrict__ counters) {
+  __shared__ uint64_t pk[2 * kSitePartials * kSiteBlock];
+  __shared__ uint32_t pw[2 * kSitePartials * L * kSiteBlock];
+  __shared__ uint32_t lcount[kExpandDigits];
+  __shared__ uint64_t gpos[kExpandDigits];
+  const uint32_t D = 1u << digit_bits;
+  for (uint32_t d = threadIdx.x; d < D; d += kSiteBlock) {
+    lcount[d] = 0;
+    gpos[d] = offsets[(uint64_t)d * gridDim.x + blockIdx.x];
+  }
+  __syncthreads();
+  uint64_t visits = 0, matched = 0;
+  uint32_t overflow = 0;
+  const uint64_t lo = (uint64_t)blockIdx.x * per_tile;
+  const uint64_t hi = lo + per_tile < S ? lo + per_tile : S;
+  for (uint64_t i = lo + threadIdx.x; i < hi; i += kSiteBlock) {
+    uint32_t w[L];
+#pragma unroll
+    for (int l = 0; l < L; ++l) w[l] = res[(uint64_t)l * res_stride + i];
+    site_walk<L, true>(keys[i], w, g, t, pr, pk, pw, visits, matched,
+                       [&](uint64_t sk, const uint32_t* v) {
+                         const uint32_t d = top_digit(sk, digit_bits);
+                         const uint64_t pos = gpos[d] + atomicAdd(&lcount[d], 1u);
+                         if (pos < cap) {
+                           Rec<L> rec;
+                           rec.key = sk;
+#pragma unroll
+                           for (int l = 0; l < L; ++l) rec.r[l] = v[l];
+                           out[pos] = rec;
+                         } else {
+                           overflow = 1;
+                         }
+                       });
+  }
+  if (overflow) atomicOr((unsigned long long*)&counters[kError], 2ull);
+}
+
 // ------------------------------------------------------------------- scan --
 
 constexpr int kScanItems = 16;
@@ -515,7 +696,8 @@ __global__ __launch_bounds__(kBlock) void k_merge_expand(
   constexpr uint32_t kCap = (T * 5) / 8;             // records per pass
   constexpr int KB = (int)((kCap + kBlock - 1) / kBlock);  // records per thread per batch
   constexpr int LL = MODE == kMergeCount ? 1 : L;   // COUNT needs no residues
-  constexpr uint32_t DM = MODE == kMergeLast ? 1 : (1u << kFineBits);
+  constexpr bool kExpand = MODE == kMergeCount || MODE == kMergeScatter;
+  constexpr uint32_t DM = kExpand ? (1u << kFineBits) : 1;
   __shared__ uint64_t tkey[T];
   __shared__ uint32_t tres[LL][T];
   __shared__ uint16_t ulist[T];
@@ -528,12 +710,12 @@ __global__ __launch_bounds__(kBlock) void k_merge_expand(
   constexpr uint32_t kLdsSlices = MODE == kMergeLast ? 1024u : 1u;
   __shared__ uint64_t ssl[kLdsSlices * L];
   const bool lds_slices = MODE == kMergeLast && slice_sums && nslices <= kLdsSlices;
-  const uint32_t D = MODE == kMergeLast ? 1u : (1u << digit_bits);
+  const uint32_t D = kExpand ? (1u << digit_bits) : 1u;
   const uint32_t W = gridDim.x;
   const int lane = threadIdx.x & 63;
 
   for (uint32_t i = threadIdx.x; i < T; i += kBlock) tkey[i] = kEmptyKey;
-  if (MODE != kMergeLast) {
+  if (kExpand) {
     for (uint32_t d = threadIdx.x; d < D; d += kBlock) {
       lcount[d] = 0;
       if (MODE == kMergeScatter) gpos[d] = offsets[(uint64_t)d * W + blockIdx.x];
@@ -696,7 +878,27 @@ __global__ __launch_bounds__(kBlock) void k_merge_expand(
             }
           }
         }
-        if (MODE != kMergeLast) {
+        if (MODE == kMergeEmit) {
+          // One global atomic per wave: lanes take consecutive positions.
+          uint64_t* ek = (uint64_t*)out;
+          uint32_t* er = (uint32_t*)(ek + cap);
+          const uint64_t act = __ballot(1);
+          const int leader = __ffsll((long long)act) - 1;
+          unsigned long long base_pos = 0;
+          if (lane == leader)
+            base_pos = atomicAdd((unsigned long long*)&counters[kEmitted], (unsigned long long)__popcll(act));
+          base_pos = __shfl(base_pos, leader, 64);
+          const uint64_t pos = base_pos + __builtin_amdgcn_mbcnt_hi((uint32_t)(act >> 32),
+                                                                    __builtin_amdgcn_mbcnt_lo((uint32_t)act, 0u));
+          if (pos < cap) {
+            ek[pos] = key;
+#pragma unroll
+            for (int l = 0; l < L; ++l) er[(uint64_t)l * cap + pos] = v[l];
+          } else {
+            overflow |= 2u;
+          }
+        }
+        if (kExpand) {
           const Board bs = unpack(key, g);
           uint32_t free = legal_positions(bs, g);
           while (free) {
@@ -838,6 +1040,40 @@ hipError_t launch_partition(int lanes, const void* in, void* out, const uint64_t
   return hipGetLastError();
 }
 
+hipError_t launch_site_count(const uint64_t* keys, uint64_t S, const RowGeom& g, const SiteTable& t,
+                             uint64_t* counters, hipStream_t stream) {
+  if (S == 0) return hipSuccess;
+  const uint32_t grid = grid_for(S, kSiteBlock, 8192);
+  hipLaunchKernelGGL(k_site_count, dim3(grid), dim3(kSiteBlock), 0, stream, keys, S, g, t, counters);
+  return hipGetLastError();
+}
+
+hipError_t launch_site_hist(const uint64_t* keys, uint64_t S, const RowGeom& g, const SiteTable& t,
+                            uint32_t digit_bits, uint32_t tiles, uint64_t per_tile, uint32_t* hist,
+                            hipStream_t stream) {
+  if (digit_bits > kFineBits || tiles == 0) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_site_hist, dim3(tiles), dim3(kSiteBlock), 0, stream, keys, S, g, t, digit_bits,
+                     per_tile, hist);
+  return hipGetLastError();
+}
+
+hipError_t launch_site_scatter(int lanes, const uint64_t* keys, const uint32_t* res, uint64_t S,
+                               uint64_t res_stride, const RowGeom& g, const SiteTable& t, const Primes& pr,
+                               uint32_t digit_bits, uint32_t tiles, uint64_t per_tile,
+                               const uint64_t* offsets, void* out, uint64_t cap, uint64_t* counters,
+                               hipStream_t stream) {
+  if (digit_bits > kFineBits || tiles == 0) return hipErrorInvalidValue;
+  dim3 grid(tiles), block(kSiteBlock);
+  switch (lanes) {
+    case 1: hipLaunchKernelGGL(k_site_scatter<1>, grid, block, 0, stream, keys, res, S, res_stride, g, t, pr, digit_bits, per_tile, offsets, (Rec<1>*)out, cap, counters); break;
+    case 2: hipLaunchKernelGGL(k_site_scatter<2>, grid, block, 0, stream, keys, res, S, res_stride, g, t, pr, digit_bits, per_tile, offsets, (Rec<2>*)out, cap, counters); break;
+    case 3: hipLaunchKernelGGL(k_site_scatter<3>, grid, block, 0, stream, keys, res, S, res_stride, g, t, pr, digit_bits, per_tile, offsets, (Rec<3>*)out, cap, counters); break;
+    case 4: hipLaunchKernelGGL(k_site_scatter<4>, grid, block, 0, stream, keys, res, S, res_stride, g, t, pr, digit_bits, per_tile, offsets, (Rec<4>*)out, cap, counters); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
 template <int L, int T>
 static hipError_t launch_me(int mode, dim3 grid, hipStream_t stream, const void* in,
                             const uint64_t* segs, uint32_t nseg, uint32_t per, const RowGeom& g,
@@ -850,6 +1086,7 @@ static hipError_t launch_me(int mode, dim3 grid, hipStream_t stream, const void*
     case kMergeCount: hipLaunchKernelGGL((k_merge_expand<L, kMergeCount, T>), grid, dim3(kBlock), 0, stream, i, segs, nseg, per, g, pr, digit_bits, hist, offsets, o, cap, final_mask, slice_sums, nslices, counters); break;
     case kMergeScatter: hipLaunchKernelGGL((k_merge_expand<L, kMergeScatter, T>), grid, dim3(kBlock), 0, stream, i, segs, nseg, per, g, pr, digit_bits, hist, offsets, o, cap, final_mask, slice_sums, nslices, counters); break;
     case kMergeLast: hipLaunchKernelGGL((k_merge_expand<L, kMergeLast, T>), grid, dim3(kBlock), 0, stream, i, segs, nseg, per, g, pr, digit_bits, hist, offsets, o, cap, final_mask, slice_sums, nslices, counters); break;
+    case kMergeEmit: hipLaunchKernelGGL((k_merge_expand<L, kMergeEmit, T>), grid, dim3(kBlock), 0, stream, i, segs, nseg, per, g, pr, digit_bits, hist, offsets, o, cap, final_mask, slice_sums, nslices, counters); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

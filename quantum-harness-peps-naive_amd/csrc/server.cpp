@@ -23,18 +23,10 @@
 #include "../../include/peps_mi355x.h"
 #include "frontend.h"
 
+using peps::fail;
 using peps::u128;
 
 namespace {
-
-int fail(int code, const std::string& msg, char* err, size_t cap) {
-  if (err && cap) {
-    const size_t n = std::min(cap - 1, msg.size());
-    std::memcpy(err, msg.data(), n);
-    err[n] = 0;
-  }
-  return code;
-}
 
 struct Request {
   uint64_t id = 0;

@@ -10,9 +10,14 @@ from .api import (  # noqa: F401
     PepsOptions,
     RequestOutcome,
     Server,
+    canonical_layer_bytes,
+    compile_row_operator,
     contract_rows,
     contract_rows_d4_orbit_sort_reduce,
+    contract_rows_hash_materialization,
+    contract_rows_sitewise,
     contract_rows_sort_reduce,
+    contract_rows_with_tensor,
     count_batched,
     count_prefix,
     crt_plan,
@@ -25,5 +30,8 @@ from .api import (  # noqa: F401
     peak_rss_bytes,
     prefix_tasks,
     profile_kernels,
+    sha256,
+    site_tensor_c,
+    trace_layers,
     version,
 )

@@ -23,6 +23,9 @@ PEPS_E_PENDING = 7
 PEPS_POLICY_SHORTEST_FIRST = 0
 PEPS_POLICY_FIFO = 1
 
+PEPS_ROW_OPERATOR_COMPILED = 0
+PEPS_ROW_OPERATOR_SITEWISE = 1
+
 PEPS_SYMMETRY_NONE = 0
 PEPS_SYMMETRY_TOP_ROW_ORBITS = 1
 
@@ -117,7 +120,8 @@ class Options(ctypes.Structure):
         ("slice_group", ctypes.c_uint32),
         ("merge_passes", ctypes.c_uint32),
         ("merge_waves", ctypes.c_uint32),
-        ("reserved", ctypes.c_uint32 * 2),
+        ("site_counters", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32 * 1),
     ]
 
 
@@ -157,4 +161,29 @@ class ServerStats(ctypes.Structure):
         ("tasks", ctypes.c_uint64),
         ("busy_s", ctypes.c_double),
         ("uptime_s", ctypes.c_double),
+    ]
+
+
+LEG_NAMES = ("column_in", "column_out", "row_in", "row_out", "diag_dr_in", "diag_dr_out", "diag_dl_in",
+             "diag_dl_out")
+
+
+class TensorEntry(ctypes.Structure):
+    """peps_tensor_entry: one entry of the explicit C tensor (CEntry, src/lib.rs:88-92)."""
+    _fields_ = [(name, ctypes.c_uint8) for name in LEG_NAMES] + [("value", ctypes.c_uint64)]
+
+    def legs(self):
+        return tuple(getattr(self, k) for k in LEG_NAMES)
+
+    def __repr__(self):
+        return f"TensorEntry(legs={self.legs()}, value={self.value})"
+
+
+class LayerDigest(ctypes.Structure):
+    _fields_ = [
+        ("row", ctypes.c_uint32),
+        ("support", ctypes.c_uint64),
+        ("weight_sum", U128),
+        ("transitions", U128),
+        ("sha256", ctypes.c_uint8 * 32),
     ]

@@ -25,10 +25,14 @@ from .abi import (
     PEPS_SYMMETRY_TOP_ROW_ORBITS,
     U128,
     ContractionResult,
+    PEPS_ROW_OPERATOR_COMPILED,
+    PEPS_ROW_OPERATOR_SITEWISE,
     KernelStat,
+    LayerDigest,
     Options,
     RequestResult,
     ServerStats,
+    TensorEntry,
 )
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -56,6 +60,7 @@ class PepsOptions:
     slice_group: int = 0        # batched form: tasks per device group (0: max)
     merge_passes: int = 0       # LDS merge passes per final bucket (0: default)
     merge_waves: int = 0        # merge grid in waves of resident workgroups (0: default)
+    site_counters: int = 0      # sitewise tensor counters: 0 indexed, 1 linear scan of C
 
     def to_c(self) -> Options:
         o = Options()
@@ -70,6 +75,7 @@ class PepsOptions:
         o.slice_group = self.slice_group
         o.merge_passes = self.merge_passes
         o.merge_waves = self.merge_waves
+        o.site_counters = self.site_counters
         return o
 
 
@@ -108,6 +114,16 @@ def load():
     L.peps_profile_kernels.argtypes = [c.c_uint32, c.c_uint32, c.c_char_p, c.c_size_t]
     L.peps_kernel_stats.argtypes = [c.c_uint32, P(KernelStat), c.c_uint32, P(c.c_uint32), c.c_char_p,
                                     c.c_size_t]
+    TE = TensorEntry
+    L.peps_site_tensor_c.argtypes = [P(TE), c.c_uint32, P(c.c_uint32)]
+    L.peps_compile_row_operator.argtypes = [P(TE), c.c_uint32, P(TE), c.c_char_p, c.c_size_t]
+    L.peps_contract_rows_sitewise.argtypes = [c.c_uint32, P(Options), P(ContractionResult), c.c_char_p,
+                                              c.c_size_t]
+    L.peps_contract_rows_with_tensor.argtypes = [c.c_uint32, P(TE), c.c_uint32, P(Options),
+                                                 P(ContractionResult), c.c_char_p, c.c_size_t]
+    L.peps_trace_layers.argtypes = [c.c_uint32, c.c_uint32, c.c_uint32, P(TE), c.c_uint32, P(Options),
+                                    P(LayerDigest), c.c_uint32, P(c.c_uint32), c.c_char_p, c.c_size_t]
+    L.peps_sha256.argtypes = [c.c_char_p, c.c_size_t, P(c.c_uint8)]
     S = c.c_void_p
     L.peps_server_create.argtypes = [P(Options), c.c_uint32, c.c_uint32, P(S), c.c_char_p, c.c_size_t]
     L.peps_server_submit.argtypes = [S, c.c_uint32, P(c.c_uint32), c.c_uint32, P(c.c_uint64), c.c_char_p,
@@ -339,3 +355,87 @@ class Server:
             self.close()
         except Exception:
             pass
+
+
+# ------------------------------------------------------------ tensor fidelity --
+
+def site_tensor_c():
+    """SiteTensorC::sec_vi() (src/lib.rs:102-139): the 17 C entries, legs_key order."""
+    arr = (TensorEntry * 64)()
+    cnt = ctypes.c_uint32()
+    load().peps_site_tensor_c(arr, 64, ctypes.byref(cnt))
+    return [TensorEntry.from_buffer_copy(arr[i]) for i in range(cnt.value)]
+
+
+def _tensor_array(entries):
+    entries = list(entries)
+    return (TensorEntry * max(1, len(entries)))(*entries), len(entries)
+
+
+def compile_row_operator(entries) -> TensorEntry:
+    """CompiledRowOperator::compile (src/lib.rs:425-485): the occupied entry, or
+    PepsError with the reference's message when C is not exactly Sec. VI's."""
+    arr, cnt = _tensor_array(entries)
+    occ = TensorEntry()
+    e = _err()
+    _check(load().peps_compile_row_operator(arr, cnt, ctypes.byref(occ), e, len(e)), e)
+    return occ
+
+
+def contract_rows_sitewise(n: int, options: PepsOptions = None) -> ContractionResult:
+    """Reference contract_rows_sitewise(n) (src/lib.rs:5629): the literal
+    sitewise C contraction, run on the GPU over HBM-resident frontiers."""
+    out = ContractionResult()
+    e = _err()
+    ref, _keep = _opts(options)
+    _check(load().peps_contract_rows_sitewise(n, ref, ctypes.byref(out), e, len(e)), e)
+    return out
+
+
+def contract_rows_with_tensor(n: int, entries, options: PepsOptions = None) -> ContractionResult:
+    """The literal sitewise contraction of a caller-supplied C (no N! bound)."""
+    arr, cnt = _tensor_array(entries)
+    out = ContractionResult()
+    e = _err()
+    ref, _keep = _opts(options)
+    _check(load().peps_contract_rows_with_tensor(n, arr, cnt, ref, ctypes.byref(out), e, len(e)), e)
+    return out
+
+
+def contract_rows_hash_materialization(n: int, options: PepsOptions = None) -> ContractionResult:
+    """Reference contract_rows_hash_materialization(n) (src/lib.rs:764-766): the
+    compiled operator without orbit slicing; the merge structure does not change
+    any reported field, so this is contract_rows_sort_reduce."""
+    return contract_rows_sort_reduce(n, options)
+
+
+def trace_layers(n: int, row_operator: str = "compiled", symmetry: bool = False, tensor=None,
+                 options: PepsOptions = None):
+    """Layer trace certificate: per row, the merged layer's support, exact
+    weight sum, transitions and the SHA-256 of its canonical serialisation
+    (states sorted by (columns, diag_dr, diag_dl, weight), 40 little-endian
+    bytes each)."""
+    op = {"compiled": PEPS_ROW_OPERATOR_COMPILED, "sitewise": PEPS_ROW_OPERATOR_SITEWISE}[row_operator]
+    arr, cnt = (None, 0) if tensor is None else _tensor_array(tensor)
+    out = (LayerDigest * 64)()
+    rows = ctypes.c_uint32()
+    e = _err()
+    ref, _keep = _opts(options)
+    sym = PEPS_SYMMETRY_TOP_ROW_ORBITS if symmetry else PEPS_SYMMETRY_NONE
+    _check(load().peps_trace_layers(n, op, sym, arr, cnt, ref, out, 64, ctypes.byref(rows), e, len(e)), e)
+    return [{"row": d.row, "support": d.support, "weight_sum": d.weight_sum.value,
+             "transitions": d.transitions.value, "sha256": bytes(d.sha256).hex()}
+            for d in out[: rows.value]]
+
+
+def sha256(data: bytes) -> str:
+    out = (ctypes.c_uint8 * 32)()
+    load().peps_sha256(data, len(data), out)
+    return bytes(out).hex()
+
+
+def canonical_layer_bytes(states) -> bytes:
+    """The trace's serialisation of a layer given as (columns, diag_dr, diag_dl, weight) tuples."""
+    import struct
+    return b"".join(struct.pack("<QQQQQ", c, dr, dl, w & ((1 << 64) - 1), w >> 64)
+                    for c, dr, dl, w in sorted(states))

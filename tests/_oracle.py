@@ -5,7 +5,7 @@ Test infrastructure only: the oracle is the checker, never the thing tested.
 import ctypes
 import os
 
-from peps_mi355x.abi import U128, ContractionResult
+from peps_mi355x.abi import U128, ContractionResult, TensorEntry
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_PATH = os.path.join(ROOT, "oracle", "libpeps_oracle.so")
@@ -13,6 +13,8 @@ LIB_PATH = os.path.join(ROOT, "oracle", "libpeps_oracle.so")
 SORT_REDUCE, HASH, SITEWISE, PARALLEL = 0, 1, 2, 3
 
 _lib = None
+LAYER_FN = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint64,
+                            ctypes.POINTER(ctypes.c_uint64))
 
 
 def lib():
@@ -42,6 +44,9 @@ def lib():
         L.oracle_row_transfer.argtypes = [c.c_uint32, c.c_uint64, c.c_uint64, c.c_uint64, c.c_uint32,
                                           c.POINTER(c.c_uint64), c.c_uint32, c.POINTER(c.c_uint32)]
         L.oracle_peak_rss_bytes.restype = c.c_uint64
+        L.oracle_contract_rows_tensor.argtypes = [c.c_uint32, c.c_uint32, c.c_uint32, c.POINTER(TensorEntry),
+                                                  c.c_uint32, c.c_uint32, LAYER_FN, c.c_void_p,
+                                                  c.POINTER(ContractionResult), c.c_char_p, c.c_size_t]
         _lib = L
     return _lib
 
@@ -153,3 +158,29 @@ def row_transfer(n, cols, dr, dl, sitewise):
     if rc:
         raise OracleError("row transfer failed")
     return [tuple(buf[4 * i: 4 * i + 4]) for i in range(cnt.value)]
+
+
+def contract_rows_tensor(n, entries=None, symmetry=False, backend=SITEWISE, linear_scan_counters=False,
+                         layers=False):
+    """Contraction with a caller-supplied C (sitewise backend) or Sec. VI's.
+
+    Returns (result, {row: sorted [(columns, diag_dr, diag_dl, weight)]}) when
+    layers is set, else the result."""
+    arr = None
+    if entries is not None:
+        arr = (TensorEntry * max(1, len(entries)))(*entries)
+    dump = {}
+
+    def cb(_user, row, count, states):
+        flat = states[: 5 * count]
+        dump[row] = sorted((flat[5 * i], flat[5 * i + 1], flat[5 * i + 2],
+                            flat[5 * i + 3] | (flat[5 * i + 4] << 64)) for i in range(count))
+
+    fn = LAYER_FN(cb) if layers else LAYER_FN()
+    out = ContractionResult()
+    e = _err()
+    rc = lib().oracle_contract_rows_tensor(n, int(symmetry), backend, arr, len(entries or ()),
+                                           int(linear_scan_counters), fn, None, ctypes.byref(out), e, 512)
+    if rc:
+        raise OracleError(e.value.decode())
+    return (out, dump) if layers else out
