@@ -174,6 +174,9 @@ def main():
     ap.add_argument("--serve-n-min", type=int, default=10)
     ap.add_argument("--serve-n-max", type=int, default=16)
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--autotune", action="store_true",
+                    help="sweep the launch shape first (bounded by --autotune-seconds) and time the best")
+    ap.add_argument("--autotune-seconds", type=float, default=240.0)
     ap.add_argument("--depth", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
@@ -193,11 +196,19 @@ def main():
     info = pm.device_info(0)
     log(f"device: {info['name']} CUs={info['cu_count']} HBM={info['hbm_bytes'] / 2**30:.0f} GiB")
 
+    opts = None
+    if args.autotune:
+        from peps_mi355x import autotune as at
+        opts, table = at.autotune(n, form=args.form, depth=args.depth, repeats=1,
+                                  budget_s=args.autotune_seconds,
+                                  progress=lambda sh, t: log(f"autotune {sh}: {t}"))
+        log(f"autotune best: {opts}")
+
     def step():
         if args.form == "plain":
-            r = pm.contract_rows(n)
+            r = pm.contract_rows(n, opts)
         else:
-            r, _ = pm.count_batched(n, args.depth)
+            r, _ = pm.count_batched(n, args.depth, opts)
         if r.count.value != pm.known_count(n):
             raise SystemExit(f"count mismatch at N={n}: {r.count.value}")
         return r
@@ -273,6 +284,7 @@ def main():
                                f"{'single merged frontier (plain drop-in)' if args.form == 'plain' else f'{args.depth}-row prefix slices (batched)'}"
                                ", row-0 orbit slicing, CRT residues",
                    "n": n, "form": args.form, "crt_lanes": last.crt_lanes,
+                   "options": ({k: v for k, v in vars(opts).items() if v} if opts else "defaults"),
                    "transitions_per_count": plain_matched},
         "time_to_count_s": min(times),
         "time_to_count_median_s": sorted(times)[len(times) // 2],
