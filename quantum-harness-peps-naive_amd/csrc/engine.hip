@@ -422,7 +422,8 @@ int Engine::pipeline(const RunSpec& spec, Plan* plan, bool eager, RunOut* out, s
   };
   // One full wave of resident SCATTER workgroups (times merge_waves), so the
   // bucket ranges finish together instead of leaving a partly occupied tail.
-  const uint64_t merge_grid = (uint64_t)cu_count_ * merge_resident_blocks(L, table_slots) *
+  const uint64_t merge_grid = (uint64_t)cu_count_ *
+                              merge_resident_blocks(L, table_slots, std::min<uint32_t>(spec.expand_bits, kFineBits)) *
                               std::max<uint32_t>(1, std::min<uint32_t>(spec.merge_waves, 8));
   auto split_bits = [&](unsigned B, unsigned d1) {
     std::vector<unsigned> d;
@@ -526,13 +527,23 @@ int Engine::pipeline(const RunSpec& spec, Plan* plan, bool eager, RunOut* out, s
     for (unsigned dk : pd) {
       used += dk;
       const int to = cur ^ 1;
+      const uint32_t chunks = partition_chunks(Crow, nseg);
+      const size_t cells = (size_t)(nseg << dk) * chunks;
       int e = grow(to, Crow * rb);
       if (!e) e = need(&segs_[sidx ^ 1], &segs_cap_[sidx ^ 1], &plan->segs_need[sidx ^ 1], (nseg << dk) + 1);
+      if (!e) e = need(&hist_, &hist_cap_, &plan->hist_need, cells);
+      if (!e) e = need(&offsets_, &offsets_cap_, &plan->offsets_need, cells);
+      if (!e) e = need(&partial_, &partial_cap_, &plan->partial_need, cells / 4096 + 2);
       if (e) return e;
-      LAUNCH("partition", 2.0 * R * Crow,
-             launch_partition(L, region_[cur], region_[to], segs_[sidx], (uint32_t)nseg, 64 - used, dk,
-                              segs_[sidx ^ 1], region_bytes_[to] / rb, row_counters(0) + n * kNumCounters,
-                              stream_));
+      uint64_t* spare = row_counters(0) + n * kNumCounters;
+      LAUNCH("part_hist", 8.0 * Crow,
+             launch_part_hist(L, region_[cur], segs_[sidx], (uint32_t)nseg, chunks, 64 - used, dk, hist_, stream_));
+      LAUNCH("scan", 12.0 * cells,
+             launch_scan(hist_, (uint32_t)(nseg << dk), chunks, 1, offsets_, segs_[sidx ^ 1], partial_, spare,
+                         stream_));
+      LAUNCH("part_scatter", 2.0 * R * Crow,
+             launch_part_scatter(L, region_[cur], region_[to], segs_[sidx], (uint32_t)nseg, chunks, 64 - used, dk,
+                                 offsets_, region_bytes_[to] / rb, spare, stream_));
       cur = to;
       sidx ^= 1;
       nseg <<= dk;
@@ -873,7 +884,7 @@ int Engine::materialized(const RunSpec& spec, RunOut* out, std::string* err) {
   const uint32_t rbits = std::max<uint32_t>(1, std::min<uint32_t>(spec.radix_bits, kMaxDigitBits));
   const uint32_t table_slots = spec.tile_states <= 1024 ? 2048u : 4096u;
   const uint64_t target = (uint64_t)std::max<uint32_t>(256, std::min<uint32_t>(spec.tile_states, merge_cap(table_slots)));
-  const uint64_t merge_grid = (uint64_t)cu_count_ * merge_resident_blocks(L, table_slots) *
+  const uint64_t merge_grid = (uint64_t)cu_count_ * merge_resident_blocks(L, table_slots, kFineBits) *
                               std::max<uint32_t>(1, std::min<uint32_t>(spec.merge_waves, 8));
 
   const size_t ncount = (size_t)(n + 1) * kNumCounters;
@@ -998,12 +1009,23 @@ int Engine::materialized(const RunSpec& spec, RunOut* out, std::string* err) {
       for (unsigned dk : pd) {
         used += dk;
         const int to = cur ^ 1;
+        const uint32_t chunks = partition_chunks(C, nseg);
+        const size_t cells = (size_t)(nseg << dk) * chunks;
         rc = ensure_region(to, C * rb, err);
         if (!rc) rc = ensure_buffer(&segs_[sidx ^ 1], &segs_cap_[sidx ^ 1], (size_t)(nseg << dk) + 1, err);
+        if (!rc) rc = ensure_buffer(&hist_, &hist_cap_, cells, err);
+        if (!rc) rc = ensure_buffer(&offsets_, &offsets_cap_, cells, err);
+        if (!rc) rc = ensure_buffer(&partial_, &partial_cap_, cells / 4096 + 2, err);
         if (rc) return rc;
-        LAUNCH("partition", 2.0 * R * C,
-               launch_partition(L, region_[cur], region_[to], segs_[sidx], (uint32_t)nseg, 64 - used, dk,
-                                segs_[sidx ^ 1], region_bytes_[to] / rb, spare, stream_));
+        LAUNCH("part_hist", 8.0 * C,
+               launch_part_hist(L, region_[cur], segs_[sidx], (uint32_t)nseg, chunks, 64 - used, dk, hist_,
+                                stream_));
+        LAUNCH("scan", 12.0 * cells,
+               launch_scan(hist_, (uint32_t)(nseg << dk), chunks, 1, offsets_, segs_[sidx ^ 1], partial_, spare,
+                           stream_));
+        LAUNCH("part_scatter", 2.0 * R * C,
+               launch_part_scatter(L, region_[cur], region_[to], segs_[sidx], (uint32_t)nseg, chunks, 64 - used,
+                                   dk, offsets_, region_bytes_[to] / rb, spare, stream_));
         cur = to;
         sidx ^= 1;
         nseg <<= dk;

@@ -24,6 +24,7 @@ constexpr int kMaxLanes = 4;
 constexpr uint64_t kEmptyKey = ~0ull;
 constexpr uint32_t kFineBits = 10;       // histogram resolution of the COUNT pass
 constexpr uint32_t kMaxDigitBits = 11;   // widest partition digit
+constexpr int kNarrowDigitBits = 8;      // SCATTER digits up to this width use smaller LDS arrays
 
 // Geometry of one row step (uniform across the launch).  Board masks are
 // 32-bit (N <= 21).  The occupied C entry's binary legs (src/lib.rs:425-485)
@@ -119,9 +120,22 @@ hipError_t launch_scan(const uint32_t* hist, uint32_t D, uint32_t W, uint32_t gr
                        uint64_t* offsets, uint64_t* segs, uint64_t* partial, uint64_t* counters,
                        hipStream_t stream);
 
-hipError_t launch_partition(int lanes, const void* in, void* out, const uint64_t* segs,
-                            uint32_t nseg, uint32_t shift, uint32_t digit_bits, uint64_t* subsegs,
-                            uint64_t cap, uint64_t* counters, hipStream_t stream);
+// One more hash-radix digit (bits [shift, shift + digit_bits) of the key mix)
+// inside each of the nseg segments of `in`, written to `out`; the scan's segs
+// output is the nseg * 2^digit_bits + 1 new segment bounds.  Every segment is
+// processed as `chunks` ranges (partition_chunks); hist and offsets need
+// nseg * 2^digit_bits * chunks entries, partial that / 4096 + 2.  A write at
+// or beyond `cap` sets bit 1 of counters[kError].
+constexpr uint64_t kPartChunk = 32768;     // target records per partition workgroup
+constexpr uint64_t kMaxPartChunks = 4096;
+uint32_t partition_chunks(uint64_t records, uint64_t nseg);
+// The partition is three launches: hist, launch_scan(hist, nseg << digit_bits,
+// chunks, 1, offsets, subsegs, partial, counters), scatter.
+hipError_t launch_part_hist(int lanes, const void* in, const uint64_t* segs, uint32_t nseg, uint32_t chunks,
+                            uint32_t shift, uint32_t digit_bits, uint32_t* hist, hipStream_t stream);
+hipError_t launch_part_scatter(int lanes, const void* in, void* out, const uint64_t* segs, uint32_t nseg,
+                               uint32_t chunks, uint32_t shift, uint32_t digit_bits, const uint64_t* offsets,
+                               uint64_t cap, uint64_t* counters, hipStream_t stream);
 
 // Fused merge of row r's buckets with the expansion into row r+1.
 // COUNT: hist[(1<<kFineBits)][W] of successor digits.  SCATTER: successors
@@ -140,7 +154,8 @@ hipError_t launch_merge_expand(int mode, int lanes, const void* in, const uint64
                                uint64_t nslices, uint64_t* counters, uint32_t table_slots,
                                hipStream_t stream);
 uint32_t merge_cap(uint32_t table_slots);
-// Workgroups of the SCATTER merge kernel that fit on one CU at once.
-uint32_t merge_resident_blocks(int lanes, uint32_t table_slots);
+// Workgroups of the SCATTER merge kernel (writing digits of up to
+// `digit_bits`) that fit on one CU at once.
+uint32_t merge_resident_blocks(int lanes, uint32_t table_slots, uint32_t digit_bits);
 
 }  // namespace peps

@@ -597,76 +597,102 @@ __global__ __launch_bounds__(kBlock) void k_scan_apply(const uint32_t* __restric
 }
 
 // -------------------------------------------------------------- partition --
-// One workgroup per segment (grid-stride).  Digit = bits [shift, shift+D) of
-// the key mix.  subsegs[s*D + d] receives the start of digit d of segment s;
-// subsegs[nseg*D] the end of the last segment.
+// One more hash-radix digit inside every segment.  Each segment is split into
+// `chunks` near-equal ranges so a launch has nseg x chunks workgroups however
+// few segments there are: k_part_hist counts the digits of every (segment,
+// digit, chunk) at hist[(s * D + d) * chunks + p] -- which is the output
+// order, so one exclusive scan gives every chunk's write position per digit
+// and the new segment table -- and k_part_scatter writes the records through
+// an LDS stage grouped by digit (the next chunk's loads in flight meanwhile).
+// PB: widest digit of the launch; the per-digit LDS arrays are sized 2^PB.
 
-constexpr uint32_t kPartDigits = 1u << kMaxDigitBits;
+__device__ __forceinline__ void chunk_range(const uint64_t* __restrict__ segs, uint32_t s, uint32_t p,
+                                            uint32_t chunks, uint64_t* a, uint64_t* b) {
+  const uint64_t lo = segs[s], len = segs[s + 1] - lo;
+  *a = lo + len * p / chunks;
+  *b = lo + len * (p + 1) / chunks;
+}
 
-template <int L>
-__global__ __launch_bounds__(kBlock) void k_partition(const Rec<L>* __restrict__ in,
-                                                      Rec<L>* __restrict__ out,
-                                                      const uint64_t* __restrict__ segs,
-                                                      uint32_t nseg, uint32_t shift,
-                                                      uint32_t digit_bits,
-                                                      uint64_t* __restrict__ subsegs, uint64_t cap,
-                                                      uint64_t* __restrict__ counters) {
-  __shared__ Stage<L, kPartDigits> st;
+template <int L, int PB>
+__global__ __launch_bounds__(kBlock) void k_part_hist(const Rec<L>* __restrict__ in,
+                                                      const uint64_t* __restrict__ segs, uint32_t nseg,
+                                                      uint32_t chunks, uint32_t shift, uint32_t digit_bits,
+                                                      uint32_t* __restrict__ hist) {
+  __shared__ uint32_t cnt[1 << PB];
   const uint32_t D = 1u << digit_bits;
   const uint64_t dmask = D - 1;
-  for (uint32_t s = blockIdx.x; s < nseg; s += gridDim.x) {
-    const uint64_t lo = segs[s], hi = segs[s + 1];
-    for (uint32_t b = threadIdx.x; b < D; b += kBlock) st.count[b] = 0;
+  const uint64_t total = (uint64_t)nseg * chunks;
+  for (uint64_t gi = blockIdx.x; gi < total; gi += gridDim.x) {
+    const uint32_t s = (uint32_t)(gi / chunks), p = (uint32_t)(gi % chunks);
+    uint64_t a, b;
+    chunk_range(segs, s, p, chunks, &a, &b);
+    for (uint32_t d = threadIdx.x; d < D; d += kBlock) cnt[d] = 0;
     lds_barrier();
-    // Sweep 1: digit counts from the keys, kPerThread loads in flight per thread.
-    for (uint64_t base = lo; base < hi; base += kStage) {
+    for (uint64_t base = a; base < b; base += kStage) {
       uint64_t k[kPerThread];
 #pragma unroll
       for (int j = 0; j < kPerThread; ++j) {
         const uint64_t i = base + (uint64_t)j * kBlock + threadIdx.x;
-        k[j] = i < hi ? in[i].key : kEmptyKey;
+        k[j] = i < b ? in[i].key : kEmptyKey;
       }
 #pragma unroll
       for (int j = 0; j < kPerThread; ++j)
-        if (k[j] != kEmptyKey) atomicAdd(&st.count[(uint32_t)((mix64(k[j]) >> shift) & dmask)], 1u);
+        if (k[j] != kEmptyKey) atomicAdd(&cnt[(uint32_t)((mix64(k[j]) >> shift) & dmask)], 1u);
     }
     lds_barrier();
-    scan_digits(st.count, st.start, D, st.wave_tot);
-    for (uint32_t b = threadIdx.x; b < D; b += kBlock) {
-      st.gpos[b] = lo + st.start[b];
-      subsegs[(uint64_t)s * D + b] = lo + st.start[b];
-      st.count[b] = 0;
+    for (uint32_t d = threadIdx.x; d < D; d += kBlock) hist[((uint64_t)s * D + d) * chunks + p] = cnt[d];
+    lds_barrier();
+  }
+}
+
+template <int L, int PB>
+__global__ __launch_bounds__(kBlock) void k_part_scatter(const Rec<L>* __restrict__ in,
+                                                         Rec<L>* __restrict__ out,
+                                                         const uint64_t* __restrict__ segs, uint32_t nseg,
+                                                         uint32_t chunks, uint32_t shift, uint32_t digit_bits,
+                                                         const uint64_t* __restrict__ offsets, uint64_t cap,
+                                                         uint64_t* __restrict__ counters) {
+  __shared__ Stage<L, (1 << PB)> st;
+  const uint32_t D = 1u << digit_bits;
+  const uint64_t dmask = D - 1;
+  const uint64_t total = (uint64_t)nseg * chunks;
+  for (uint64_t gi = blockIdx.x; gi < total; gi += gridDim.x) {
+    const uint32_t s = (uint32_t)(gi / chunks), p = (uint32_t)(gi % chunks);
+    uint64_t a, b;
+    chunk_range(segs, s, p, chunks, &a, &b);
+    for (uint32_t d = threadIdx.x; d < D; d += kBlock) {
+      st.count[d] = 0;
+      st.gpos[d] = offsets[((uint64_t)s * D + d) * chunks + p];
     }
     lds_barrier();
-    // Sweep 2: the next chunk is loaded while the current one is flushed.
     Rec<L> nx[kPerThread];
 #pragma unroll
     for (int j = 0; j < kPerThread; ++j) {
-      const uint64_t i = lo + (uint64_t)j * kBlock + threadIdx.x;
-      if (i < hi) nx[j] = in[i];
+      const uint64_t i = a + (uint64_t)j * kBlock + threadIdx.x;
+      if (i < b) nx[j] = in[i];
     }
-    for (uint64_t base = lo; base < hi; base += kStage) {
-      const uint32_t nrec = (hi - base < (uint64_t)kStage) ? (uint32_t)(hi - base) : (uint32_t)kStage;
+    for (uint64_t base = a; base < b; base += kStage) {
+      const uint32_t nrec = (b - base < (uint64_t)kStage) ? (uint32_t)(b - base) : (uint32_t)kStage;
       Rec<L> cur[kPerThread];
 #pragma unroll
       for (int j = 0; j < kPerThread; ++j) cur[j] = nx[j];
       const uint64_t nb = base + kStage;
-      if (nb < hi) {
+      if (nb < b) {
 #pragma unroll
         for (int j = 0; j < kPerThread; ++j) {
           const uint64_t i = nb + (uint64_t)j * kBlock + threadIdx.x;
-          if (i < hi) nx[j] = in[i];
+          if (i < b) nx[j] = in[i];
         }
       }
 #pragma unroll
       for (int j = 0; j < kPerThread; ++j) {
         const uint32_t slot = (uint32_t)j * kBlock + threadIdx.x;
-        if (slot < nrec) stage_put<L, kPartDigits>(st, slot, cur[j], (uint32_t)((mix64(cur[j].key) >> shift) & dmask));
+        if (slot < nrec)
+          stage_put<L, (1 << PB)>(st, slot, cur[j], (uint32_t)((mix64(cur[j].key) >> shift) & dmask));
       }
-      stage_flush<L, kPartDigits>(st, nrec, D, out, cap, counters);
+      stage_flush<L, (1 << PB)>(st, nrec, D, out, cap, counters);
     }
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0) subsegs[(uint64_t)nseg * D] = segs[nseg];
 }
 
 // ----------------------------------------------------------- merge-expand --
@@ -686,7 +712,9 @@ __global__ __launch_bounds__(kBlock) void k_partition(const Rec<L>* __restrict__
 // the keys whose mix falls in one class (bits 12-35), so passes see disjoint
 // keys.  Probe exhaustion sets kError; the call then fails.
 
-template <int L, int MODE, int T>
+// DB: widest next-row digit the kernel can write (LDS per-digit arrays are
+// sized 2^DB; SCATTER at <= 8 bits keeps four workgroups per CU).
+template <int L, int MODE, int T, int DB = kFineBits>
 __global__ __launch_bounds__(kBlock) void k_merge_expand(
     const Rec<L>* __restrict__ in, const uint64_t* __restrict__ segs, uint32_t nseg, uint32_t per,
     RowGeom g, Primes pr, uint32_t digit_bits, uint32_t* __restrict__ hist,
@@ -697,7 +725,7 @@ __global__ __launch_bounds__(kBlock) void k_merge_expand(
   constexpr int KB = (int)((kCap + kBlock - 1) / kBlock);  // records per thread per batch
   constexpr int LL = MODE == kMergeCount ? 1 : L;   // COUNT needs no residues
   constexpr bool kExpand = MODE == kMergeCount || MODE == kMergeScatter;
-  constexpr uint32_t DM = kExpand ? (1u << kFineBits) : 1;
+  constexpr uint32_t DM = kExpand ? (1u << DB) : 1;
   __shared__ uint64_t tkey[T];
   __shared__ uint32_t tres[LL][T];
   __shared__ uint16_t ulist[T];
@@ -1025,18 +1053,61 @@ hipError_t launch_scan(const uint32_t* hist, uint32_t D, uint32_t W, uint32_t gr
   return hipGetLastError();
 }
 
-hipError_t launch_partition(int lanes, const void* in, void* out, const uint64_t* segs,
-                            uint32_t nseg, uint32_t shift, uint32_t digit_bits, uint64_t* subsegs,
-                            uint64_t cap, uint64_t* counters, hipStream_t stream) {
-  if (digit_bits == 0 || digit_bits > kMaxDigitBits || nseg == 0) return hipErrorInvalidValue;
-  dim3 grid(nseg < 8192 ? nseg : 8192), block(kBlock);
+uint32_t partition_chunks(uint64_t records, uint64_t nseg) {
+  const uint64_t per = nseg * kPartChunk;
+  const uint64_t c = (records + per - 1) / per;
+  return (uint32_t)(c < 1 ? 1 : (c > kMaxPartChunks ? kMaxPartChunks : c));
+}
+
+hipError_t launch_part_hist(int lanes, const void* in, const uint64_t* segs, uint32_t nseg, uint32_t chunks,
+                            uint32_t shift, uint32_t digit_bits, uint32_t* hist, hipStream_t stream) {
+  if (digit_bits == 0 || digit_bits > kMaxDigitBits || nseg == 0 || chunks == 0) return hipErrorInvalidValue;
+  if (((uint64_t)nseg << digit_bits) > 0xffffffffull) return hipErrorInvalidValue;
+  const uint64_t total = (uint64_t)nseg * chunks;
+  dim3 grid(total < 65536 ? (uint32_t)total : 65536u), block(kBlock);
+  const bool narrow = digit_bits <= (uint32_t)kNarrowDigitBits;
+#define PEPS_PH(LN)                                                                                           \
+  if (narrow)                                                                                                 \
+    hipLaunchKernelGGL((k_part_hist<LN, kNarrowDigitBits>), grid, block, 0, stream, (const Rec<LN>*)in, segs, \
+                       nseg, chunks, shift, digit_bits, hist);                                               \
+  else                                                                                                        \
+    hipLaunchKernelGGL((k_part_hist<LN, kMaxDigitBits>), grid, block, 0, stream, (const Rec<LN>*)in, segs,    \
+                       nseg, chunks, shift, digit_bits, hist);                                               \
+  break
   switch (lanes) {
-    case 1: hipLaunchKernelGGL(k_partition<1>, grid, block, 0, stream, (const Rec<1>*)in, (Rec<1>*)out, segs, nseg, shift, digit_bits, subsegs, cap, counters); break;
-    case 2: hipLaunchKernelGGL(k_partition<2>, grid, block, 0, stream, (const Rec<2>*)in, (Rec<2>*)out, segs, nseg, shift, digit_bits, subsegs, cap, counters); break;
-    case 3: hipLaunchKernelGGL(k_partition<3>, grid, block, 0, stream, (const Rec<3>*)in, (Rec<3>*)out, segs, nseg, shift, digit_bits, subsegs, cap, counters); break;
-    case 4: hipLaunchKernelGGL(k_partition<4>, grid, block, 0, stream, (const Rec<4>*)in, (Rec<4>*)out, segs, nseg, shift, digit_bits, subsegs, cap, counters); break;
+    case 1: PEPS_PH(1);
+    case 2: PEPS_PH(2);
+    case 3: PEPS_PH(3);
+    case 4: PEPS_PH(4);
     default: return hipErrorInvalidValue;
   }
+#undef PEPS_PH
+  return hipGetLastError();
+}
+
+hipError_t launch_part_scatter(int lanes, const void* in, void* out, const uint64_t* segs, uint32_t nseg,
+                               uint32_t chunks, uint32_t shift, uint32_t digit_bits, const uint64_t* offsets,
+                               uint64_t cap, uint64_t* counters, hipStream_t stream) {
+  if (digit_bits == 0 || digit_bits > kMaxDigitBits || nseg == 0 || chunks == 0) return hipErrorInvalidValue;
+  const uint64_t total = (uint64_t)nseg * chunks;
+  dim3 grid(total < 65536 ? (uint32_t)total : 65536u), block(kBlock);
+  const bool narrow = digit_bits <= (uint32_t)kNarrowDigitBits;
+#define PEPS_PS(LN)                                                                                            \
+  if (narrow)                                                                                                  \
+    hipLaunchKernelGGL((k_part_scatter<LN, kNarrowDigitBits>), grid, block, 0, stream, (const Rec<LN>*)in,     \
+                       (Rec<LN>*)out, segs, nseg, chunks, shift, digit_bits, offsets, cap, counters);          \
+  else                                                                                                         \
+    hipLaunchKernelGGL((k_part_scatter<LN, kMaxDigitBits>), grid, block, 0, stream, (const Rec<LN>*)in,        \
+                       (Rec<LN>*)out, segs, nseg, chunks, shift, digit_bits, offsets, cap, counters);          \
+  break
+  switch (lanes) {
+    case 1: PEPS_PS(1);
+    case 2: PEPS_PS(2);
+    case 3: PEPS_PS(3);
+    case 4: PEPS_PS(4);
+    default: return hipErrorInvalidValue;
+  }
+#undef PEPS_PS
   return hipGetLastError();
 }
 
@@ -1084,7 +1155,12 @@ static hipError_t launch_me(int mode, dim3 grid, hipStream_t stream, const void*
   Rec<L>* o = (Rec<L>*)out;
   switch (mode) {
     case kMergeCount: hipLaunchKernelGGL((k_merge_expand<L, kMergeCount, T>), grid, dim3(kBlock), 0, stream, i, segs, nseg, per, g, pr, digit_bits, hist, offsets, o, cap, final_mask, slice_sums, nslices, counters); break;
-    case kMergeScatter: hipLaunchKernelGGL((k_merge_expand<L, kMergeScatter, T>), grid, dim3(kBlock), 0, stream, i, segs, nseg, per, g, pr, digit_bits, hist, offsets, o, cap, final_mask, slice_sums, nslices, counters); break;
+    case kMergeScatter:
+      if (digit_bits <= kNarrowDigitBits)
+        hipLaunchKernelGGL((k_merge_expand<L, kMergeScatter, T, kNarrowDigitBits>), grid, dim3(kBlock), 0, stream, i, segs, nseg, per, g, pr, digit_bits, hist, offsets, o, cap, final_mask, slice_sums, nslices, counters);
+      else
+        hipLaunchKernelGGL((k_merge_expand<L, kMergeScatter, T>), grid, dim3(kBlock), 0, stream, i, segs, nseg, per, g, pr, digit_bits, hist, offsets, o, cap, final_mask, slice_sums, nslices, counters);
+      break;
     case kMergeLast: hipLaunchKernelGGL((k_merge_expand<L, kMergeLast, T>), grid, dim3(kBlock), 0, stream, i, segs, nseg, per, g, pr, digit_bits, hist, offsets, o, cap, final_mask, slice_sums, nslices, counters); break;
     case kMergeEmit: hipLaunchKernelGGL((k_merge_expand<L, kMergeEmit, T>), grid, dim3(kBlock), 0, stream, i, segs, nseg, per, g, pr, digit_bits, hist, offsets, o, cap, final_mask, slice_sums, nslices, counters); break;
     default: return hipErrorInvalidValue;
@@ -1094,10 +1170,12 @@ static hipError_t launch_me(int mode, dim3 grid, hipStream_t stream, const void*
 
 uint32_t merge_cap(uint32_t table_slots) { return (table_slots * 5) / 8; }
 
-uint32_t merge_resident_blocks(int lanes, uint32_t table_slots) {
+uint32_t merge_resident_blocks(int lanes, uint32_t table_slots, uint32_t digit_bits) {
   int blocks = 0;
   const void* f = nullptr;
-#define PEPS_KF(LN, TS) f = (const void*)&k_merge_expand<LN, kMergeScatter, TS>
+#define PEPS_KF(LN, TS)                                                                  \
+  f = digit_bits <= kNarrowDigitBits ? (const void*)&k_merge_expand<LN, kMergeScatter, TS, kNarrowDigitBits> \
+                                     : (const void*)&k_merge_expand<LN, kMergeScatter, TS>
   if (table_slots == 2048) {
     switch (lanes) {
       case 1: PEPS_KF(1, 2048); break;
