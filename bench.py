@@ -186,6 +186,9 @@ def main():
     ap.add_argument("--serve-n-min", type=int, default=10)
     ap.add_argument("--serve-n-max", type=int, default=16)
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--lanes", type=int, default=0, help="CRT lanes (0: the plan for N)")
+    ap.add_argument("--graph", action="store_true",
+                    help="record the size plan on the warmup and replay each timed step as HIP graphs")
     ap.add_argument("--autotune", action="store_true",
                     help="sweep the launch shape first (bounded by --autotune-seconds) and time the best")
     ap.add_argument("--autotune-seconds", type=float, default=240.0)
@@ -215,6 +218,10 @@ def main():
                                   budget_s=args.autotune_seconds,
                                   progress=lambda sh, t: log(f"autotune {sh}: {t}"))
         log(f"autotune best: {opts}")
+    if args.lanes or args.graph:
+        opts = opts or pm.PepsOptions()
+        opts.crt_lanes = args.lanes
+        opts.graph_mode = 1 if args.graph else 0
 
     def step():
         if args.form == "plain":
