@@ -10,6 +10,8 @@
 #include <algorithm>
 #include <chrono>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -178,6 +180,36 @@ int peps_count_batched(uint32_t n, uint32_t split_depth, const peps_options* opt
     done += g1 - g0;
     if (g_progress) g_progress(g_progress_user, done, tasks.size());
   };
+  // Graph mode: the grouping found by the first call is kept; the next call
+  // records every group's size plan and later calls replay the whole batched
+  // count as one HIP graph.
+  static std::mutex cache_mu;
+  static std::map<std::string, std::vector<std::pair<size_t, size_t>>> grouping;
+  char keybuf[256];
+  snprintf(keybuf, sizeof(keybuf), "batched/%u/%u/%zu/%u/%u/%u/%u/%u/%u/%llu/%u", n, split_depth, r.primes.size(),
+           r.radix_bits, r.tile_states, r.expand_bits, r.merge_passes, r.merge_waves, r.slice_group,
+           (unsigned long long)r.budget, r.device);
+  const std::string key = keybuf;
+  std::vector<std::pair<size_t, size_t>> ranges;
+  bool cached = false;
+  if (r.graph) {
+    std::lock_guard<std::mutex> lk(cache_mu);
+    auto it = grouping.find(key);
+    if (it != grouping.end()) {
+      ranges = it->second;
+      cached = true;
+    }
+  }
+  if (cached) {
+    rc = peps::run_task_groups(r, n, split_depth, tasks, ranges, key, absorb, &e);
+    if (rc) return fail(rc, e, err, err_cap);
+    if (overflow) return fail(PEPS_E_OVERFLOW, "batched count overflow", err, err_cap);
+    next = tasks.size();
+  }
+  auto record = [&](size_t g0, size_t g1, const peps::RunOut& ro) {
+    ranges.emplace_back(g0, g1);
+    absorb(g0, g1, ro);
+  };
   while (next < tasks.size()) {
     // Admit tasks while their projected candidates fit 80% of a buffer.
     const double ratio = proj.ratio(n, split_depth);
@@ -193,9 +225,13 @@ int peps_count_batched(uint32_t n, uint32_t split_depth, const peps_options* opt
       ++g1;
     }
     next = g1;
-    rc = peps::run_task_range(r, n, split_depth, tasks, g0, g1, absorb, &e);
+    rc = peps::run_task_range(r, n, split_depth, tasks, g0, g1, record, &e);
     if (rc) return fail(rc, e, err, err_cap);
     if (overflow) return fail(PEPS_E_OVERFLOW, "batched count overflow", err, err_cap);
+    if (next == tasks.size() && r.graph) {
+      std::lock_guard<std::mutex> lk(cache_mu);
+      grouping[key] = ranges;
+    }
   }
   total.layers = layers;
   u128 fact;

@@ -119,6 +119,12 @@ class Engine {
  public:
   static Engine* get(int device, std::string* err);  // one engine per device
   int run(const RunSpec& spec, RunOut* out, std::string* err);
+  // Runs the fused pipelines of `specs` one after another.  The first call
+  // with a given `key` runs them eagerly and records every size plan; later
+  // calls with the same key (and the same specs) replay all of them as one
+  // HIP graph, each group with its own seeds, counters and slice sums.
+  int run_many(const std::string& key, const std::vector<RunSpec>& specs, std::vector<RunOut>* outs,
+               std::string* err);
   std::mutex& mutex() { return mu_; }
   int device() const { return device_; }
   // Records one candidate buffer can hold (0 before the first run sets the budget).
@@ -139,7 +145,22 @@ class Engine {
   std::mutex mu_;
   hipStream_t stream_ = nullptr;
   hipEvent_t ev_start_ = nullptr, ev_stop_ = nullptr;
+  int prepare(const RunSpec& spec, std::string* err);
   int pipeline(const RunSpec& spec, Plan* plan, bool eager, RunOut* out, std::string* err);
+  struct GroupBufs {
+    uint64_t* seed = nullptr;
+    uint64_t* counters = nullptr;
+    uint64_t* slices = nullptr;
+    size_t seed_words = 0, counter_words = 0, slice_words = 0;
+  };
+  struct BatchGraph {
+    std::vector<Plan> plans;
+    std::vector<GroupBufs> bufs;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    uint64_t generation = 0;
+  };
+  std::map<std::string, BatchGraph> batches_;
   int materialized(const RunSpec& spec, RunOut* out, std::string* err);
   int finish(const RunSpec& spec, const Plan& plan, RunOut* out, std::string* err);
   std::map<std::string, Plan> plans_;

@@ -259,6 +259,168 @@ std::string plan_key(const RunSpec& s) {
 
 }  // namespace
 
+int Engine::prepare(const RunSpec& spec, std::string* err) {
+  HIP_TRY(hipSetDevice(device_));
+  const unsigned n = spec.n;
+  const int L = (int)spec.primes.size();
+  if (n == 0 || n > 21 || L < 1 || L > kMaxLanes) {
+    *err = "engine: unsupported board or lane count";
+    return 1;
+  }
+  if (3 * n + spec.slice_bits > 63 || spec.slice_bits > 20) {
+    *err = "engine: slice ids do not fit above the 3N-bit boundary key";
+    return 1;
+  }
+  if (spec.first_row >= n || spec.seeds.empty()) {
+    *err = "engine: nothing to contract";
+    return 1;
+  }
+  if (budget_ == 0 || (spec.budget_bytes && spec.budget_bytes != budget_)) {
+    if (spec.budget_bytes) {
+      budget_ = spec.budget_bytes;
+    } else {
+      size_t free_b = 0, total_b = 0;
+      HIP_TRY(hipMemGetInfo(&free_b, &total_b));
+      budget_ = (uint64_t)((double)(free_b + arena_bytes()) * 0.85);
+    }
+  }
+  return 0;
+}
+
+int Engine::run_many(const std::string& key, const std::vector<RunSpec>& specs, std::vector<RunOut>* outs,
+                     std::string* err) {
+  outs->assign(specs.size(), RunOut{});
+  for (const RunSpec& sp : specs) {
+    if (sp.materialize || sp.site) {
+      *err = "engine: grouped graph replay covers the fused row pipeline only";
+      return 1;
+    }
+    const int rc = prepare(sp, err);
+    if (rc) return rc;
+  }
+  auto it = batches_.find(key);
+  if (it == batches_.end() || it->second.plans.size() != specs.size()) {
+    // First call: eager runs, each recording its size plan.
+    BatchGraph bg;
+    bg.plans.resize(specs.size());
+    for (size_t i = 0; i < specs.size(); ++i) {
+      const auto t0 = Clock::now();
+      const double a0 = alloc_s_;
+      const int rc = pipeline(specs[i], &bg.plans[i], true, &(*outs)[i], err);
+      if (rc) return rc;
+      (*outs)[i].alloc_s = alloc_s_ - a0;
+      (*outs)[i].elapsed_s = std::chrono::duration<double>(Clock::now() - t0).count();
+    }
+    batches_[key] = std::move(bg);
+    return 0;
+  }
+  BatchGraph& bg = it->second;
+  const int L = (int)specs[0].primes.size();
+  const size_t L8 = 8 + 4 * (size_t)L;
+  // Shared buffers at least as large as every recorded group needs.
+  int rc = 0;
+  for (const Plan& plan : bg.plans) {
+    if (!rc) rc = ensure_region(0, plan.region_need, err);
+    if (!rc) rc = ensure_region(1, plan.region_need, err);
+    if (!rc) rc = ensure_buffer(&hist_, &hist_cap_, plan.hist_need, err);
+    if (!rc) rc = ensure_buffer(&offsets_, &offsets_cap_, plan.offsets_need, err);
+    if (!rc) rc = ensure_buffer(&partial_, &partial_cap_, plan.partial_need, err);
+    if (!rc) rc = ensure_buffer(&segs_[0], &segs_cap_[0], plan.segs_need[0], err);
+    if (!rc) rc = ensure_buffer(&segs_[1], &segs_cap_[1], plan.segs_need[1], err);
+  }
+  if (rc) return rc;
+  // Per-group seeds, counters and slice sums, so that every group's pipeline
+  // can sit in one graph.
+  if (bg.bufs.empty()) {
+    bg.bufs.resize(specs.size());
+    for (size_t i = 0; i < specs.size(); ++i) {
+      const RunSpec& sp = specs[i];
+      GroupBufs& gb = bg.bufs[i];
+      const uint64_t S0 = sp.seeds.size();
+      gb.seed_words = (S0 * L8 + 7) / 8;
+      gb.counter_words = (size_t)(sp.n + 1) * kNumCounters;
+      gb.slice_words = sp.per_slice ? ((size_t)1 << sp.slice_bits) * L : 0;
+      HIP_TRY(hipMalloc((void**)&gb.seed, gb.seed_words * 8));
+      HIP_TRY(hipMalloc((void**)&gb.counters, gb.counter_words * 8));
+      if (gb.slice_words) HIP_TRY(hipMalloc((void**)&gb.slices, gb.slice_words * 8));
+      std::vector<uint64_t> host(gb.seed_words, 0);
+      uint32_t* res = (uint32_t*)(host.data() + S0);
+      for (uint64_t j = 0; j < S0; ++j) {
+        host[j] = sp.seeds[j].key;
+        for (int l = 0; l < L; ++l) res[(size_t)l * S0 + j] = (uint32_t)(sp.seeds[j].weight % sp.primes[l]);
+      }
+      HIP_TRY(hipMemcpy(gb.seed, host.data(), gb.seed_words * 8, hipMemcpyHostToDevice));
+    }
+  }
+  struct Swap {
+    Engine* e;
+    uint64_t *seed, *counters, *slices;
+    size_t seed_cap, counters_cap, slice_cap;
+    Swap(Engine* en, const GroupBufs& gb)
+        : e(en), seed(en->seed_), counters(en->counters_rows_), slices(en->slice_sums_),
+          seed_cap(en->seed_cap_), counters_cap(en->counters_rows_cap_), slice_cap(en->slice_cap_) {
+      e->seed_ = gb.seed;
+      e->seed_cap_ = gb.seed_words;
+      e->counters_rows_ = gb.counters;
+      e->counters_rows_cap_ = gb.counter_words;
+      e->slice_sums_ = gb.slices;
+      e->slice_cap_ = gb.slice_words;
+    }
+    ~Swap() {
+      e->seed_ = seed;
+      e->seed_cap_ = seed_cap;
+      e->counters_rows_ = counters;
+      e->counters_rows_cap_ = counters_cap;
+      e->slice_sums_ = slices;
+      e->slice_cap_ = slice_cap;
+    }
+  };
+  if (bg.exec && bg.generation != generation_) {
+    HIP_TRY(hipGraphExecDestroy(bg.exec));
+    HIP_TRY(hipGraphDestroy(bg.graph));
+    bg.exec = nullptr;
+    bg.graph = nullptr;
+  }
+  if (!bg.exec) {
+    HIP_TRY(hipStreamSynchronize(stream_));
+    const bool prof = profiling_;
+    profiling_ = false;
+    HIP_TRY(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+    for (size_t i = 0; i < specs.size() && !rc; ++i) {
+      Swap sw(this, bg.bufs[i]);
+      RunOut scratch;
+      rc = pipeline(specs[i], &bg.plans[i], false, &scratch, err);
+    }
+    hipGraph_t graph = nullptr;
+    const hipError_t ec = hipStreamEndCapture(stream_, &graph);
+    profiling_ = prof;
+    if (rc) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return rc;
+    }
+    HIP_TRY(ec);
+    HIP_TRY(hipGraphInstantiate(&bg.exec, graph, nullptr, nullptr, 0));
+    bg.graph = graph;
+    bg.generation = generation_;
+  }
+  const auto t0 = Clock::now();
+  HIP_TRY(hipEventRecord(ev_start_, stream_));
+  HIP_TRY(hipGraphLaunch(bg.exec, stream_));
+  HIP_TRY(hipEventRecord(ev_stop_, stream_));
+  HIP_TRY(hipEventSynchronize(ev_stop_));
+  float ms = 0;
+  HIP_TRY(hipEventElapsedTime(&ms, ev_start_, ev_stop_));
+  for (size_t i = 0; i < specs.size(); ++i) {
+    Swap sw(this, bg.bufs[i]);
+    rc = finish(specs[i], bg.plans[i], &(*outs)[i], err);
+    if (rc) return rc;
+    (*outs)[i].replayed = true;
+  }
+  (*outs)[0].kernel_s = ms / 1000.0;
+  (*outs)[0].elapsed_s = std::chrono::duration<double>(Clock::now() - t0).count();
+  return 0;
+}
+
 int Engine::run(const RunSpec& spec, RunOut* out, std::string* err) {
   const auto t0 = Clock::now();
   const double alloc0 = alloc_s_;

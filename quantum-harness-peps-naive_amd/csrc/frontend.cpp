@@ -145,17 +145,21 @@ int check_board(unsigned n, std::string* err) {
   return PEPS_OK;
 }
 
+void apply_resolved(const Resolved& r, RunSpec* spec) {
+  spec->primes = r.primes;
+  spec->radix_bits = r.radix_bits;
+  spec->tile_states = r.tile_states;
+  spec->expand_bits = r.expand_bits;
+  spec->use_graph = r.graph;
+  spec->merge_passes = r.merge_passes;
+  spec->merge_waves = r.merge_waves;
+  spec->budget_bytes = r.budget;
+}
+
 int run_engine(const Resolved& r, RunSpec& spec, RunOut* ro, std::string* err) {
   Engine* eng = Engine::get((int)r.device, err);
   if (!eng) return PEPS_E_DEVICE;
-  spec.primes = r.primes;
-  spec.radix_bits = r.radix_bits;
-  spec.tile_states = r.tile_states;
-  spec.expand_bits = r.expand_bits;
-  spec.use_graph = r.graph;
-  spec.merge_passes = r.merge_passes;
-  spec.merge_waves = r.merge_waves;
-  spec.budget_bytes = r.budget;
+  apply_resolved(r, &spec);
   std::lock_guard<std::mutex> lk(eng->mutex());
   return eng->run(spec, ro, err);
 }
@@ -258,6 +262,40 @@ size_t max_slices(unsigned n, uint32_t slice_group) {
   return std::min<size_t>((size_t)1 << free_bits, slice_group ? slice_group : SIZE_MAX);
 }
 
+RunSpec task_spec(unsigned n, unsigned depth, const std::vector<Task>& tasks, size_t a, size_t b) {
+  RunSpec spec;
+  spec.n = n;
+  spec.first_row = depth;
+  spec.orbit_first_row = depth == 0;
+  unsigned bits = 0;
+  while (((size_t)1 << bits) < b - a) ++bits;
+  spec.slice_bits = bits;
+  spec.per_slice = true;
+  for (size_t i = a; i < b; ++i)
+    spec.seeds.push_back(Seed{tasks[i].key | ((uint64_t)(i - a) << (3 * n)), tasks[i].weight});
+  return spec;
+}
+
+int run_task_groups(const Resolved& r, unsigned n, unsigned depth, const std::vector<Task>& tasks,
+                    const std::vector<std::pair<size_t, size_t>>& groups, const std::string& key,
+                    const std::function<void(size_t, size_t, const RunOut&)>& done, std::string* err) {
+  Engine* eng = Engine::get((int)r.device, err);
+  if (!eng) return PEPS_E_DEVICE;
+  std::vector<RunSpec> specs;
+  for (const auto& [a, b] : groups) {
+    specs.push_back(task_spec(n, depth, tasks, a, b));
+    apply_resolved(r, &specs.back());
+  }
+  std::vector<RunOut> outs;
+  {
+    std::lock_guard<std::mutex> lk(eng->mutex());
+    const int rc = eng->run_many(key, specs, &outs, err);
+    if (rc) return rc;
+  }
+  for (size_t i = 0; i < groups.size(); ++i) done(groups[i].first, groups[i].second, outs[i]);
+  return PEPS_OK;
+}
+
 int run_task_range(const Resolved& r, unsigned n, unsigned depth, const std::vector<Task>& tasks,
                    size_t g0, size_t g1,
                    const std::function<void(size_t, size_t, const RunOut&)>& done,
@@ -266,16 +304,7 @@ int run_task_range(const Resolved& r, unsigned n, unsigned depth, const std::vec
   while (!work.empty()) {
     const auto [a, b] = work.back();
     work.pop_back();
-    RunSpec spec;
-    spec.n = n;
-    spec.first_row = depth;
-    spec.orbit_first_row = depth == 0;
-    unsigned bits = 0;
-    while (((size_t)1 << bits) < b - a) ++bits;
-    spec.slice_bits = bits;
-    spec.per_slice = true;
-    for (size_t i = a; i < b; ++i)
-      spec.seeds.push_back(Seed{tasks[i].key | ((uint64_t)(i - a) << (3 * n)), tasks[i].weight});
+    RunSpec spec = task_spec(n, depth, tasks, a, b);
     RunOut ro;
     std::string e;
     const int rc = run_engine(r, spec, &ro, &e);

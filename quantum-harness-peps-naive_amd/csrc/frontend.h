@@ -97,6 +97,16 @@ class Projection {
 // Largest number of tasks one launch can carry as slices (ids above bit 3N).
 size_t max_slices(unsigned n, uint32_t slice_group);
 
+// The spec that runs tasks[a, b) as slices (slice id i - a above bit 3N).
+RunSpec task_spec(unsigned n, unsigned depth, const std::vector<Task>& tasks, size_t a, size_t b);
+
+// Runs the given task groups, each as the slices of one pipeline, through
+// Engine::run_many: the first call with `key` records every group's size
+// plan, later calls replay all groups as one HIP graph.
+int run_task_groups(const Resolved& r, unsigned n, unsigned depth, const std::vector<Task>& tasks,
+                    const std::vector<std::pair<size_t, size_t>>& groups, const std::string& key,
+                    const std::function<void(size_t, size_t, const RunOut&)>& done, std::string* err);
+
 // Runs tasks[g0, g1) as the slices of one launch starting at row `depth`; a
 // launch that exceeds the HBM budget is split in halves.  `done` is called
 // once per successful launch with its range and output; `slice_counts` of that

@@ -183,8 +183,28 @@ def test_batched_task_counts_match_dfs():
 
 @pytest.mark.parametrize("group", [1, 5, 0])
 def test_batched_grouping_and_graph(group):
+    # graph_mode: call 1 finds the grouping, call 2 records every group's size
+    # plan, calls 3+ replay all groups as one HIP graph.
+    ref = None
     for graph in (0, 1):
-        for _ in range(2):
+        for rep in range(4 if graph else 2):
+            show("batched graph", group, graph, rep)
             b, ntasks, counts = pm.count_batched(14, 2, pm.PepsOptions(slice_group=group, graph_mode=graph),
                                                  task_counts=True)
             assert b.count.value == 365596 and sum(counts) == 365596
+            layers = [tuple(l[k] for k in KEYS) for l in b.layer_list()]
+            if ref is None:
+                ref = (counts, layers)
+            assert (counts, layers) == ref, (graph, rep)
+
+
+def test_batched_graph_replay_interleaved():
+    # Replays of one batched shape between other calls that reallocate.
+    o_ = pm.PepsOptions(graph_mode=1)
+    want = [pm.count_batched(13, 2, task_counts=True)[2], pm.count_batched(15, 2, task_counts=True)[2]]
+    for rep in range(3):
+        for i, n in enumerate((13, 15)):
+            show("batched interleave", n, rep)
+            b, _, counts = pm.count_batched(n, 2, o_, task_counts=True)
+            assert counts == want[i] and b.count.value == o.known_count(n)
+        pm.contract_rows(16)
