@@ -121,7 +121,7 @@ typedef struct {
   uint32_t slice_group;      /* batched form: prefix tasks per device group (0: as many as
                                 the slice-id bits above the 3N-bit key allow) */
   uint32_t merge_passes;     /* LDS merge passes per final bucket (0: auto; 1..8) */
-  uint32_t merge_waves;      /* merge grid in waves of resident workgroups (0: 4; 1..8) */
+  uint32_t merge_waves;      /* merge grid in waves of resident workgroups (0: 8; 1..8) */
   uint32_t site_counters;    /* sitewise tensor counters: 0 indexed (src/lib.rs:537-540),
                                 1 every C entry per partial-row visit (benchmarks/n14_layers.csv) */
   uint32_t reserved[1];

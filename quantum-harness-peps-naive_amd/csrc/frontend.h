@@ -46,7 +46,7 @@ struct Resolved {
   bool graph = false;
   uint32_t slice_group = 0;
   uint32_t merge_passes = 1;
-  uint32_t merge_waves = 4;
+  uint32_t merge_waves = 8;
   uint32_t site_counters = 0;
 };
 
