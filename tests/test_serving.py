@@ -168,3 +168,29 @@ def test_open_loop_stream_counts():
         if key not in memo:
             memo[key] = expected(n, cols)
         assert r.count == memo[key], key
+
+
+@pytest.mark.gpu
+def test_concurrent_submitters_and_direct_calls():
+    # Four threads submit while the main thread makes direct library calls on
+    # the same engine; every result must still be exact.
+    import threading
+    reqs = sv.make_requests(seed=7, count=60, rate=1e6, n_min=8, n_max=13)
+    results = {}
+    with pm.Server() as s:
+        def worker(part):
+            for idx in part:
+                _, n, cols = reqs[idx]
+                results[idx] = s.submit(n, cols)
+        threads = [threading.Thread(target=worker, args=(range(t, len(reqs), 4),)) for t in range(4)]
+        for t in threads:
+            t.start()
+        direct = [pm.contract_rows(n).count.value for n in (9, 11, 12)]
+        for t in threads:
+            t.join()
+        outs = {idx: s.wait(t, 120) for idx, t in results.items()}
+    assert direct == [o.known_count(n) for n in (9, 11, 12)]
+    assert len(outs) == len(reqs)
+    for idx, r in outs.items():
+        _, n, cols = reqs[idx]
+        assert r.status == 0 and r.count == expected(n, cols), (n, cols)
