@@ -30,6 +30,7 @@ from .api import (  # noqa: F401
     peak_rss_bytes,
     prefix_tasks,
     profile_kernels,
+    set_progress,
     sha256,
     site_tensor_c,
     trace_layers,

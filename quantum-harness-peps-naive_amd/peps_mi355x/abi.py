@@ -30,6 +30,10 @@ PEPS_SYMMETRY_NONE = 0
 PEPS_SYMMETRY_TOP_ROW_ORBITS = 1
 
 
+# peps_progress_fn: (user, done, total)
+PROGRESS_FN = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint64)
+
+
 class U128(ctypes.Structure):
     _fields_ = [("lo", ctypes.c_uint64), ("hi", ctypes.c_uint64)]
 

@@ -23,6 +23,7 @@ from .abi import (
     PEPS_POLICY_SHORTEST_FIRST,
     PEPS_SYMMETRY_NONE,
     PEPS_SYMMETRY_TOP_ROW_ORBITS,
+    PROGRESS_FN,
     U128,
     ContractionResult,
     PEPS_ROW_OPERATOR_COMPILED,
@@ -114,6 +115,7 @@ def load():
     L.peps_profile_kernels.argtypes = [c.c_uint32, c.c_uint32, c.c_char_p, c.c_size_t]
     L.peps_kernel_stats.argtypes = [c.c_uint32, P(KernelStat), c.c_uint32, P(c.c_uint32), c.c_char_p,
                                     c.c_size_t]
+    L.peps_set_progress.argtypes = [PROGRESS_FN, c.c_void_p]
     TE = TensorEntry
     L.peps_site_tensor_c.argtypes = [P(TE), c.c_uint32, P(c.c_uint32)]
     L.peps_compile_row_operator.argtypes = [P(TE), c.c_uint32, P(TE), c.c_char_p, c.c_size_t]
@@ -439,3 +441,14 @@ def canonical_layer_bytes(states) -> bytes:
     import struct
     return b"".join(struct.pack("<QQQQQ", c, dr, dl, w & ((1 << 64) - 1), w >> 64)
                     for c, dr, dl, w in sorted(states))
+
+
+_progress_keep = None
+
+
+def set_progress(callback) -> None:
+    """callback(done, total) after every task group of the batched form (None: off)."""
+    global _progress_keep
+    fn = PROGRESS_FN(lambda _u, done, total: callback(done, total)) if callback else PROGRESS_FN()
+    _progress_keep = fn
+    load().peps_set_progress(fn, None)
