@@ -198,8 +198,8 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
-              f"(the contraction runs on a single MI355X; see the sharding note in DESIGN.md)",
+        print(f"bench.py: --gpus {args.gpus} is not supported: the contraction runs on one MI355X "
+              f"and only the one-GPU throughput is measured",
               file=sys.stderr)
         sys.exit(2)
 
