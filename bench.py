@@ -31,9 +31,9 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, vendor figure
 
 # rocprofv3 kernel names of the engine's profiling labels (L lanes, T slots).
 ROCPROF_NAMES = {
-    "merge_count": "k_merge_expand<{L}, 0, {T}, 10>",
-    "merge_scatter": "k_merge_expand<{L}, 1, {T}, 8>",
-    "merge_last": "k_merge_expand<{L}, 2, {T}, 10>",
+    "merge_count": "k_merge_expand<{L}, 0, {T}, 10, 256>",
+    "merge_scatter": "k_merge_expand<{L}, 1, {T}, 8, 512>",
+    "merge_last": "k_merge_expand<{L}, 2, {T}, 10, 256>",
     "part_hist": "k_part_hist<{L}, 8>",
     "part_scatter": "k_part_scatter<{L}, 8>",
 }

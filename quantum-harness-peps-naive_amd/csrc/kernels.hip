@@ -30,6 +30,7 @@ constexpr int kBlock = 256;           // threads per workgroup (4 wave64s)
 constexpr int kWaves = kBlock / 64;
 constexpr int kStage = 2048;          // LDS-staged records per flush
 constexpr int kPerThread = kStage / kBlock;
+constexpr int kScatterBlock = 512;    // SCATTER merge workgroup: 8 wave64s share one LDS table
 
 template <int L>
 struct alignas(8) Rec {
@@ -142,6 +143,7 @@ __device__ __forceinline__ uint64_t block_exclusive_scan64(uint64_t v, uint64_t*
   return before + x - v;
 }
 
+template <int NW = kWaves>
 __device__ __forceinline__ uint64_t block_sum_u64(uint64_t v, uint64_t* scratch) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
@@ -150,7 +152,7 @@ __device__ __forceinline__ uint64_t block_sum_u64(uint64_t v, uint64_t* scratch)
   lds_barrier();
   uint64_t s = 0;
 #pragma unroll
-  for (int w = 0; w < kWaves; ++w) s += scratch[w];
+  for (int w = 0; w < NW; ++w) s += scratch[w];
   lds_barrier();
   return s;
 }
@@ -714,15 +716,16 @@ __global__ __launch_bounds__(kBlock) void k_part_scatter(const Rec<L>* __restric
 
 // DB: widest next-row digit the kernel can write (LDS per-digit arrays are
 // sized 2^DB; SCATTER at <= 8 bits keeps four workgroups per CU).
-template <int L, int MODE, int T, int DB = kFineBits>
-__global__ __launch_bounds__(kBlock) void k_merge_expand(
+template <int L, int MODE, int T, int DB = kFineBits, int BT = kBlock>
+__global__ __launch_bounds__(BT, (BT == kScatterBlock && L <= 2 && T == 2048 && DB <= kNarrowDigitBits ? 8 : 1)) void k_merge_expand(
     const Rec<L>* __restrict__ in, const uint64_t* __restrict__ segs, uint32_t nseg, uint32_t per,
     RowGeom g, Primes pr, uint32_t digit_bits, uint32_t* __restrict__ hist,
     const uint64_t* __restrict__ offsets, Rec<L>* __restrict__ out, uint64_t cap,
     uint64_t final_mask, uint64_t* __restrict__ slice_sums, uint64_t nslices,
     uint64_t* __restrict__ counters) {
   constexpr uint32_t kCap = (T * 5) / 8;             // records per pass
-  constexpr int KB = (int)((kCap + kBlock - 1) / kBlock);  // records per thread per batch
+  constexpr int KB = (int)((kCap + BT - 1) / BT);  // records per thread per batch
+  constexpr uint32_t kSpan = (uint32_t)KB * BT;  // records per batch (>= kCap)
   constexpr int LL = MODE == kMergeCount ? 1 : L;   // COUNT needs no residues
   constexpr bool kExpand = MODE == kMergeCount || MODE == kMergeScatter;
   constexpr uint32_t DM = kExpand ? (1u << DB) : 1;
@@ -731,7 +734,7 @@ __global__ __launch_bounds__(kBlock) void k_merge_expand(
   __shared__ uint16_t ulist[T];
   __shared__ uint32_t lcount[DM];
   __shared__ uint64_t gpos[MODE == kMergeScatter ? DM : 1];
-  __shared__ uint64_t scratch[kWaves];
+  __shared__ uint64_t scratch[BT / 64];
   __shared__ uint32_t nuniq[2];
   // Last row: per-slice sums gathered in LDS, flushed with one global atomic
   // per (slice, lane) and workgroup.
@@ -742,16 +745,16 @@ __global__ __launch_bounds__(kBlock) void k_merge_expand(
   const uint32_t W = gridDim.x;
   const int lane = threadIdx.x & 63;
 
-  for (uint32_t i = threadIdx.x; i < T; i += kBlock) tkey[i] = kEmptyKey;
+  for (uint32_t i = threadIdx.x; i < T; i += BT) tkey[i] = kEmptyKey;
   if (kExpand) {
-    for (uint32_t d = threadIdx.x; d < D; d += kBlock) {
+    for (uint32_t d = threadIdx.x; d < D; d += BT) {
       lcount[d] = 0;
       if (MODE == kMergeScatter) gpos[d] = offsets[(uint64_t)d * W + blockIdx.x];
     }
   }
   if (threadIdx.x < 2) nuniq[threadIdx.x] = 0;
   if (lds_slices)
-    for (uint32_t i = threadIdx.x; i < nslices * L; i += kBlock) ssl[i] = 0;
+    for (uint32_t i = threadIdx.x; i < nslices * L; i += BT) ssl[i] = 0;
   lds_barrier();
 
   uint64_t uniq = 0;
@@ -768,7 +771,7 @@ __global__ __launch_bounds__(kBlock) void k_merge_expand(
   auto load_batch = [&](uint64_t base, uint64_t hi) {
 #pragma unroll
     for (int k = 0; k < KB; ++k) {
-      const uint64_t i = base + (uint64_t)k * kBlock + threadIdx.x;
+      const uint64_t i = base + (uint64_t)k * BT + threadIdx.x;
       if (i < hi) {
         if (MODE == kMergeCount)
           buf[k].key = in[i].key;
@@ -795,7 +798,7 @@ __global__ __launch_bounds__(kBlock) void k_merge_expand(
     }
     for (uint32_t pass = 0; pass < passes; ++pass) {
       uint32_t* const nu_ctr = &nuniq[parity];
-      for (uint64_t base = lo; base < hi; base += kCap) {
+      for (uint64_t base = lo; base < hi; base += kSpan) {
         if (passes > 1) load_batch(base, hi);
         // ---- 1. insert keys; owners append their slot to the unique list
         uint32_t own = 0, dup = 0;
@@ -803,7 +806,7 @@ __global__ __launch_bounds__(kBlock) void k_merge_expand(
 #pragma unroll
         for (int k = 0; k < KB; ++k) {
           sl[k] = 0;
-          const uint64_t i = base + (uint64_t)k * kBlock + threadIdx.x;
+          const uint64_t i = base + (uint64_t)k * BT + threadIdx.x;
           if (i >= hi) continue;
           const uint64_t key = buf[k].key;
           const uint64_t h = mix64(key);
@@ -877,7 +880,7 @@ __global__ __launch_bounds__(kBlock) void k_merge_expand(
       if (pass + 1 == passes && b + 1 < b1 && nhi - nlo <= kCap) load_batch(nlo, nhi);
       // ---- 4. statistics and expansion of the merged states
       const uint32_t nu = *nu_ctr;
-      for (uint32_t u = threadIdx.x; u < nu; u += kBlock) {
+      for (uint32_t u = threadIdx.x; u < nu; u += BT) {
         const uint32_t slot = ulist[u];
         const uint64_t key = tkey[slot];
         tkey[slot] = kEmptyKey;
@@ -961,23 +964,23 @@ __global__ __launch_bounds__(kBlock) void k_merge_expand(
 
   // ---- flush statistics
   if (MODE == kMergeCount) {
-    for (uint32_t d = threadIdx.x; d < D; d += kBlock) hist[(uint64_t)d * W + blockIdx.x] = lcount[d];
+    for (uint32_t d = threadIdx.x; d < D; d += BT) hist[(uint64_t)d * W + blockIdx.x] = lcount[d];
   } else {
-    const uint64_t u = block_sum_u64(uniq, scratch);
+    const uint64_t u = block_sum_u64<BT / 64>(uniq, scratch);
     if (threadIdx.x == 0 && u) atomicAdd((unsigned long long*)&counters[kOutStates], (unsigned long long)u);
 #pragma unroll
     for (int l = 0; l < L; ++l) {
-      const uint64_t w = block_sum_u64(wsum[l] % pr.p[l], scratch);
+      const uint64_t w = block_sum_u64<BT / 64>(wsum[l] % pr.p[l], scratch);
       if (threadIdx.x == 0 && w) atomicAdd((unsigned long long*)&counters[kWeight0 + l], (unsigned long long)(w % pr.p[l]));
       {
-        const uint64_t f = block_sum_u64(fsum[l] % pr.p[l], scratch);
+        const uint64_t f = block_sum_u64<BT / 64>(fsum[l] % pr.p[l], scratch);
         if (threadIdx.x == 0 && f) atomicAdd((unsigned long long*)&counters[kFinal0 + l], (unsigned long long)(f % pr.p[l]));
       }
     }
   }
   if (lds_slices) {
     lds_barrier();
-    for (uint32_t i = threadIdx.x; i < nslices * L; i += kBlock)
+    for (uint32_t i = threadIdx.x; i < nslices * L; i += BT)
       if (ssl[i]) atomicAdd((unsigned long long*)&slice_sums[i], (unsigned long long)ssl[i]);
   }
   if (overflow) atomicOr((unsigned long long*)&counters[kError], (unsigned long long)overflow);
@@ -1157,9 +1160,9 @@ static hipError_t launch_me(int mode, dim3 grid, hipStream_t stream, const void*
     case kMergeCount: hipLaunchKernelGGL((k_merge_expand<L, kMergeCount, T>), grid, dim3(kBlock), 0, stream, i, segs, nseg, per, g, pr, digit_bits, hist, offsets, o, cap, final_mask, slice_sums, nslices, counters); break;
     case kMergeScatter:
       if (digit_bits <= kNarrowDigitBits)
-        hipLaunchKernelGGL((k_merge_expand<L, kMergeScatter, T, kNarrowDigitBits>), grid, dim3(kBlock), 0, stream, i, segs, nseg, per, g, pr, digit_bits, hist, offsets, o, cap, final_mask, slice_sums, nslices, counters);
+        hipLaunchKernelGGL((k_merge_expand<L, kMergeScatter, T, kNarrowDigitBits, kScatterBlock>), grid, dim3(kScatterBlock), 0, stream, i, segs, nseg, per, g, pr, digit_bits, hist, offsets, o, cap, final_mask, slice_sums, nslices, counters);
       else
-        hipLaunchKernelGGL((k_merge_expand<L, kMergeScatter, T>), grid, dim3(kBlock), 0, stream, i, segs, nseg, per, g, pr, digit_bits, hist, offsets, o, cap, final_mask, slice_sums, nslices, counters);
+        hipLaunchKernelGGL((k_merge_expand<L, kMergeScatter, T, kFineBits, kScatterBlock>), grid, dim3(kScatterBlock), 0, stream, i, segs, nseg, per, g, pr, digit_bits, hist, offsets, o, cap, final_mask, slice_sums, nslices, counters);
       break;
     case kMergeLast: hipLaunchKernelGGL((k_merge_expand<L, kMergeLast, T>), grid, dim3(kBlock), 0, stream, i, segs, nseg, per, g, pr, digit_bits, hist, offsets, o, cap, final_mask, slice_sums, nslices, counters); break;
     case kMergeEmit: hipLaunchKernelGGL((k_merge_expand<L, kMergeEmit, T>), grid, dim3(kBlock), 0, stream, i, segs, nseg, per, g, pr, digit_bits, hist, offsets, o, cap, final_mask, slice_sums, nslices, counters); break;
@@ -1174,8 +1177,8 @@ uint32_t merge_resident_blocks(int lanes, uint32_t table_slots, uint32_t digit_b
   int blocks = 0;
   const void* f = nullptr;
 #define PEPS_KF(LN, TS)                                                                  \
-  f = digit_bits <= kNarrowDigitBits ? (const void*)&k_merge_expand<LN, kMergeScatter, TS, kNarrowDigitBits> \
-                                     : (const void*)&k_merge_expand<LN, kMergeScatter, TS>
+  f = digit_bits <= kNarrowDigitBits ? (const void*)&k_merge_expand<LN, kMergeScatter, TS, kNarrowDigitBits, kScatterBlock> \
+                                     : (const void*)&k_merge_expand<LN, kMergeScatter, TS, kFineBits, kScatterBlock>
   if (table_slots == 2048) {
     switch (lanes) {
       case 1: PEPS_KF(1, 2048); break;
@@ -1192,7 +1195,7 @@ uint32_t merge_resident_blocks(int lanes, uint32_t table_slots, uint32_t digit_b
     }
   }
 #undef PEPS_KF
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, f, kBlock, 0) != hipSuccess || blocks < 1) blocks = 1;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, f, kScatterBlock, 0) != hipSuccess || blocks < 1) blocks = 1;
   return (uint32_t)blocks;
 }
 
