@@ -29,7 +29,8 @@ sys.path.insert(0, os.path.join(ROOT, "quantum-harness-peps-naive_amd"))
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, vendor figure
 
-# rocprofv3 kernel names of the engine's profiling labels (L lanes, T slots).
+# rocprofv3 kernel names of the engine's profiling labels (L: CRT lanes of
+# the record, 0 for packed 12-byte records; T: LDS table slots).
 ROCPROF_NAMES = {
     "merge_count": "k_merge_expand<{L}, 0, {T}, 10, 256>",
     "merge_scatter": "k_merge_expand<{L}, 1, {T}, 8, 512>",
@@ -37,6 +38,11 @@ ROCPROF_NAMES = {
     "part_hist": "k_part_hist<{L}, 8>",
     "part_scatter": "k_part_scatter<{L}, 8>",
 }
+
+
+def record_tag(result):
+    """Template argument L of the record type a run used."""
+    return 0 if result.record_bytes == 12 else result.crt_lanes
 
 
 def measured_traffic(n, form, kernel, lanes, table_slots=2048):
@@ -288,7 +294,7 @@ def main():
         achieved = s["algorithmic_bytes"] / s["seconds"] / 1e9
         roofline = {"bound": "hbm", "kernel": name, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                    "traffic": measured_traffic(n, args.form, name, last.crt_lanes),
+                    "traffic": measured_traffic(n, args.form, name, record_tag(last)),
                     "avg_launch_us": round(1e6 * s["seconds"] / s["launches"], 3),
                     "algorithmic_bytes_per_launch": s["algorithmic_bytes"] / s["launches"]}
 
@@ -310,12 +316,15 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "u32",
+        "dtype": "u64" if last.record_bytes == 12 else "u32",
         "data": "synthetic (deterministic: the empty N x N board)",
         "config": {"workload": f"exact N-Queens count, N={n}, "
                                f"{'single merged frontier (plain drop-in)' if args.form == 'plain' else f'{args.depth}-row prefix slices (batched)'}"
-                               ", row-0 orbit slicing, CRT residues",
+                               ", row-0 orbit slicing, "
+                               + ("exact u64 weights in packed 12-byte records" if last.record_bytes == 12
+                                  else "u32 CRT residue records"),
                    "n": n, "form": args.form, "crt_lanes": last.crt_lanes,
+                   "record_bytes": last.record_bytes,
                    "options": ({k: v for k, v in vars(opts).items() if v} if opts else "defaults"),
                    "transitions_per_count": plain_matched, "transitions_source": transitions_source},
         "time_to_count_s": min(times),

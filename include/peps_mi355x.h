@@ -97,6 +97,7 @@ typedef struct {
   peps_layer_metric layers[PEPS_MAX_N];
   /* device extras */
   uint32_t crt_lanes;
+  uint32_t record_bytes; /* candidate record size the solve used (12 packed, 16 or 24 lanes) */
   uint64_t primes[PEPS_MAX_LANES];
   uint64_t residues[PEPS_MAX_LANES];
   uint64_t device_peak_bytes;
@@ -124,8 +125,18 @@ typedef struct {
   uint32_t merge_waves;      /* merge grid in waves of resident workgroups (0: 8; 1..8) */
   uint32_t site_counters;    /* sitewise tensor counters: 0 indexed (src/lib.rs:537-540),
                                 1 every C entry per partial-row visit (benchmarks/n14_layers.csv) */
-  uint32_t reserved[1];
+  uint32_t record_format;    /* low byte, PEPS_RECORDS_*: candidate record format of the fused
+                                row pipeline; bits 8..15, if nonzero, narrow the packed weight
+                                field to that many bits (exercises the overflow path) */
 } peps_options;
+
+/* Candidate records of the fused row pipeline.  AUTO: packed 12-byte records
+ * (the 3N-bit key and slice id, then the exact integer weight in the
+ * remaining bits, at most 52), repeated with LANES records if a weight
+ * outgrows its field.  LANES: the key and one u32 residue per CRT prime (16
+ * bytes for 1-2 primes, 24 for 3-4).  PACKED: packed only; a weight overflow
+ * fails with PEPS_E_OVERFLOW.  Counts are identical in every format. */
+enum { PEPS_RECORDS_AUTO = 0, PEPS_RECORDS_LANES = 1, PEPS_RECORDS_PACKED = 2 };
 
 /* ---------------------------------------------------------------- solve -- */
 

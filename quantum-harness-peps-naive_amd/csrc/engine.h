@@ -1,11 +1,12 @@
 // engine.h -- device-resident row contraction driver (one per device).
 //
 // The engine owns a HIP stream and an HBM arena of two candidate buffers
-// whose roles alternate every row.  The seed frontier (SoA: u64 keys, then one
-// u32 residue array per CRT prime) is expanded into the first row's
-// hash-radix buckets; from then on each row's buckets are merged and expanded
-// straight into the next row's buckets (merge_expand), so the frontier is
-// never materialised in HBM.
+// whose roles alternate every row.  The seed frontier (SoA: u64 keys, then
+// the exact u64 weights for packed records or one u32 residue array per CRT
+// prime for lane records) is expanded into the first row's hash-radix
+// buckets; from then on each row's buckets are merged and expanded straight
+// into the next row's buckets (merge_expand), so the frontier is never
+// materialised in HBM.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -25,6 +26,13 @@ namespace peps {
 struct Seed {
   uint64_t key;  // packed boundary, slice id above bit 3n
   u128 weight;
+};
+
+// Candidate record format of the fused pipeline.
+enum RecordFormat : int {
+  kRecordsAuto = 0,    // packed 12-byte records; lane records if a weight outgrows them
+  kRecordsLanes = 1,   // key + one u32 residue per CRT prime (16 or 24 bytes)
+  kRecordsPacked = 2,  // packed only: a weight overflow fails the run
 };
 
 struct RunSpec {
@@ -48,6 +56,8 @@ struct RunSpec {
   bool materialize = false;
   const SiteTable* site = nullptr;
   bool bound_check = true;         // per-row sums and the count must not exceed N!
+  int records = kRecordsAuto;      // fused pipeline record format (materialised: lanes)
+  uint32_t packed_weight_bits = kPackedMaxWeightBits;  // narrower packed weights (tests)
   // Materialised mode: called after every row with the merged layer
   // (keys[S], residues[l * S + i]); a nonzero return aborts the run.
   std::function<int(unsigned row, const std::vector<uint64_t>& keys, const std::vector<uint32_t>& res,
@@ -81,6 +91,7 @@ struct RunOut {
   double elapsed_s = 0, kernel_s = 0;
   uint64_t device_peak_bytes = 0;
   bool replayed = false;           // the run was a HIP-graph replay of a recorded plan
+  uint32_t record_bytes = 0;       // candidate record size the run used
   double alloc_s = 0;              // host time spent growing device buffers
 };
 
@@ -118,6 +129,8 @@ struct KernelStat {
 class Engine {
  public:
   static Engine* get(int device, std::string* err);  // one engine per device
+  // A run in kRecordsAuto format whose weights outgrow the packed records is
+  // repeated with lane records (same result, larger records).
   int run(const RunSpec& spec, RunOut* out, std::string* err);
   // Runs the fused pipelines of `specs` one after another.  The first call
   // with a given `key` runs them eagerly and records every size plan; later
@@ -146,6 +159,9 @@ class Engine {
   hipStream_t stream_ = nullptr;
   hipEvent_t ev_start_ = nullptr, ev_stop_ = nullptr;
   int prepare(const RunSpec& spec, std::string* err);
+  int run_once(const RunSpec& spec, RunOut* out, std::string* err);
+  int run_many_once(const std::string& key, const std::vector<RunSpec>& specs, std::vector<RunOut>* outs,
+                    std::string* err);
   int pipeline(const RunSpec& spec, Plan* plan, bool eager, RunOut* out, std::string* err);
   struct GroupBufs {
     uint64_t* seed = nullptr;
@@ -183,7 +199,7 @@ class Engine {
   size_t partial_cap_ = 0;
   uint64_t* counters_rows_ = nullptr;  // per-row counters, (n + 1) * kNumCounters
   size_t counters_rows_cap_ = 0;
-  uint64_t* seed_ = nullptr;           // seed frontier (SoA)
+  uint64_t* seed_ = nullptr;           // seed frontier (SoA, seed_image layout)
   size_t seed_cap_ = 0;
   uint64_t* front_ = nullptr;          // materialised frontier (SoA), u64 units
   size_t front_cap_ = 0;

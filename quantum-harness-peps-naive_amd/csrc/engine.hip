@@ -238,6 +238,49 @@ int Engine::resolve_pending(std::string* err) {
 
 namespace {
 
+// Internal return code: a packed record's weight outgrew its field.
+constexpr int kRcPackedOverflow = 100;
+
+bool packed_records(const RunSpec& s) { return s.records != kRecordsLanes && !s.materialize && !s.site; }
+
+// Key and weight widths of the packed records of a run.
+void packed_widths(const RunSpec& s, uint32_t* key_bits, uint32_t* weight_bits) {
+  *key_bits = 3 * s.n + s.slice_bits;
+  *weight_bits = std::min<uint32_t>(std::min<uint32_t>(96 - *key_bits, kPackedMaxWeightBits),
+                                    std::max<uint32_t>(1, s.packed_weight_bits));
+}
+
+// u64 words of the device seed image.
+size_t seed_words(const RunSpec& s) {
+  const size_t S0 = s.seeds.size();
+  return packed_records(s) ? 2 * S0 : (S0 * (8 + 4 * s.primes.size()) + 7) / 8;
+}
+
+// Device image of the seed frontier: keys[S0], then the exact u64 weights
+// (packed records) or one u32 residue array per CRT prime (lane records).
+int seed_image(const RunSpec& s, std::vector<uint64_t>* host) {
+  const size_t S0 = s.seeds.size();
+  host->assign(seed_words(s), 0);
+  uint64_t* keys = host->data();
+  if (packed_records(s)) {
+    uint32_t kb, wb;
+    packed_widths(s, &kb, &wb);
+    for (size_t i = 0; i < S0; ++i) {
+      keys[i] = s.seeds[i].key;
+      if (s.seeds[i].weight >> wb) return kRcPackedOverflow;
+      keys[S0 + i] = (uint64_t)s.seeds[i].weight;
+    }
+    return 0;
+  }
+  uint32_t* res = (uint32_t*)(keys + S0);
+  for (size_t i = 0; i < S0; ++i) {
+    keys[i] = s.seeds[i].key;
+    for (size_t l = 0; l < s.primes.size(); ++l)
+      res[l * S0 + i] = (uint32_t)(s.seeds[i].weight % s.primes[l]);
+  }
+  return 0;
+}
+
 // Identity of a run for the plan cache: everything that determines sizes.
 std::string plan_key(const RunSpec& s) {
   uint64_t h = 1469598103934665603ull;  // FNV-1a over the seeds
@@ -250,10 +293,10 @@ std::string plan_key(const RunSpec& s) {
     }
   }
   char buf[256];
-  snprintf(buf, sizeof(buf), "%u/%u/%d/%u/%zu/%u/%u/%u/%u/%u/%d/%zu/%llx", s.n, s.first_row,
+  snprintf(buf, sizeof(buf), "%u/%u/%d/%u/%zu/%u/%u/%u/%u/%u/%d/%zu/%llx/%d/%u", s.n, s.first_row,
            (int)s.orbit_first_row, s.slice_bits, s.primes.size(), s.tile_states, s.radix_bits,
            s.expand_bits, s.merge_passes, s.merge_waves, (int)s.per_slice, s.seeds.size(),
-           (unsigned long long)h);
+           (unsigned long long)h, s.records, s.packed_weight_bits);
   return buf;
 }
 
@@ -289,6 +332,20 @@ int Engine::prepare(const RunSpec& spec, std::string* err) {
 
 int Engine::run_many(const std::string& key, const std::vector<RunSpec>& specs, std::vector<RunOut>* outs,
                      std::string* err) {
+  const int rc = run_many_once(key, specs, outs, err);
+  if (rc != kRcPackedOverflow) return rc;
+  for (const RunSpec& sp : specs)
+    if (sp.records == kRecordsPacked) {
+      *err = "packed record weight overflow";
+      return 2;
+    }
+  std::vector<RunSpec> lanes = specs;
+  for (RunSpec& sp : lanes) sp.records = kRecordsLanes;
+  return run_many_once(key + "/lanes", lanes, outs, err);
+}
+
+int Engine::run_many_once(const std::string& key, const std::vector<RunSpec>& specs, std::vector<RunOut>* outs,
+                          std::string* err) {
   outs->assign(specs.size(), RunOut{});
   for (const RunSpec& sp : specs) {
     if (sp.materialize || sp.site) {
@@ -315,8 +372,6 @@ int Engine::run_many(const std::string& key, const std::vector<RunSpec>& specs, 
     return 0;
   }
   BatchGraph& bg = it->second;
-  const int L = (int)specs[0].primes.size();
-  const size_t L8 = 8 + 4 * (size_t)L;
   // Shared buffers at least as large as every recorded group needs.
   int rc = 0;
   for (const Plan& plan : bg.plans) {
@@ -336,19 +391,14 @@ int Engine::run_many(const std::string& key, const std::vector<RunSpec>& specs, 
     for (size_t i = 0; i < specs.size(); ++i) {
       const RunSpec& sp = specs[i];
       GroupBufs& gb = bg.bufs[i];
-      const uint64_t S0 = sp.seeds.size();
-      gb.seed_words = (S0 * L8 + 7) / 8;
+      std::vector<uint64_t> host;
+      if (seed_image(sp, &host)) return kRcPackedOverflow;
+      gb.seed_words = host.size();
       gb.counter_words = (size_t)(sp.n + 1) * kNumCounters;
-      gb.slice_words = sp.per_slice ? ((size_t)1 << sp.slice_bits) * L : 0;
+      gb.slice_words = sp.per_slice ? ((size_t)1 << sp.slice_bits) * sp.primes.size() : 0;
       HIP_TRY(hipMalloc((void**)&gb.seed, gb.seed_words * 8));
       HIP_TRY(hipMalloc((void**)&gb.counters, gb.counter_words * 8));
       if (gb.slice_words) HIP_TRY(hipMalloc((void**)&gb.slices, gb.slice_words * 8));
-      std::vector<uint64_t> host(gb.seed_words, 0);
-      uint32_t* res = (uint32_t*)(host.data() + S0);
-      for (uint64_t j = 0; j < S0; ++j) {
-        host[j] = sp.seeds[j].key;
-        for (int l = 0; l < L; ++l) res[(size_t)l * S0 + j] = (uint32_t)(sp.seeds[j].weight % sp.primes[l]);
-      }
       HIP_TRY(hipMemcpy(gb.seed, host.data(), gb.seed_words * 8, hipMemcpyHostToDevice));
     }
   }
@@ -422,6 +472,18 @@ int Engine::run_many(const std::string& key, const std::vector<RunSpec>& specs, 
 }
 
 int Engine::run(const RunSpec& spec, RunOut* out, std::string* err) {
+  const int rc = run_once(spec, out, err);
+  if (rc != kRcPackedOverflow) return rc;
+  if (spec.records == kRecordsPacked) {
+    *err = "packed record weight overflow";
+    return 2;
+  }
+  RunSpec lanes = spec;
+  lanes.records = kRecordsLanes;
+  return run_once(lanes, out, err);
+}
+
+int Engine::run_once(const RunSpec& spec, RunOut* out, std::string* err) {
   const auto t0 = Clock::now();
   const double alloc0 = alloc_s_;
   struct AllocReport {
@@ -469,7 +531,8 @@ int Engine::run(const RunSpec& spec, RunOut* out, std::string* err) {
       Plan& plan = it->second;
       // Buffers at least as large as the recorded run; any reallocation
       // invalidates a captured graph (it holds device pointers).
-      const size_t L8 = 8 + 4 * (size_t)L;
+      std::vector<uint64_t> seed_host;
+      if (seed_image(spec, &seed_host)) return kRcPackedOverflow;
       rc = ensure_region(0, plan.region_need, err);
       if (!rc) rc = ensure_region(1, plan.region_need, err);
       if (!rc) rc = ensure_buffer(&hist_, &hist_cap_, plan.hist_need, err);
@@ -478,7 +541,7 @@ int Engine::run(const RunSpec& spec, RunOut* out, std::string* err) {
       if (!rc) rc = ensure_buffer(&segs_[0], &segs_cap_[0], plan.segs_need[0], err);
       if (!rc) rc = ensure_buffer(&segs_[1], &segs_cap_[1], plan.segs_need[1], err);
       if (!rc) rc = ensure_buffer(&counters_rows_, &counters_rows_cap_, (size_t)(n + 1) * kNumCounters, err);
-      if (!rc) rc = ensure_buffer(&seed_, &seed_cap_, (spec.seeds.size() * L8 + 7) / 8, err);
+      if (!rc) rc = ensure_buffer(&seed_, &seed_cap_, seed_host.size(), err);
       if (!rc && spec.per_slice)
         rc = ensure_buffer(&slice_sums_, &slice_cap_, ((size_t)1 << spec.slice_bits) * L, err);
       if (rc) return rc;
@@ -507,16 +570,7 @@ int Engine::run(const RunSpec& spec, RunOut* out, std::string* err) {
         plan.generation = generation_;
       }
       // Seeds are uploaded outside the graph, then the graph replays.
-      {
-        const uint64_t S0 = spec.seeds.size();
-        std::vector<uint64_t> host((S0 * L8 + 7) / 8, 0);
-        uint32_t* res = (uint32_t*)(host.data() + S0);
-        for (uint64_t i = 0; i < S0; ++i) {
-          host[i] = spec.seeds[i].key;
-          for (int l = 0; l < L; ++l) res[(size_t)l * S0 + i] = (uint32_t)(spec.seeds[i].weight % spec.primes[l]);
-        }
-        HIP_TRY(hipMemcpy(seed_, host.data(), S0 * L8, hipMemcpyHostToDevice));
-      }
+      HIP_TRY(hipMemcpy(seed_, seed_host.data(), seed_host.size() * 8, hipMemcpyHostToDevice));
       HIP_TRY(hipEventRecord(ev_start_, stream_));
       HIP_TRY(hipGraphLaunch(plan.exec, stream_));
       HIP_TRY(hipEventRecord(ev_stop_, stream_));
@@ -551,10 +605,15 @@ int Engine::run(const RunSpec& spec, RunOut* out, std::string* err) {
 int Engine::pipeline(const RunSpec& spec, Plan* plan, bool eager, RunOut* out, std::string* err) {
   const unsigned n = spec.n;
   const int L = (int)spec.primes.size();
-  const size_t rb = record_bytes(L);
-  const double R = 8.0 + 4.0 * L;  // algorithmic bytes per record: key + L residues
+  // Record layout of the candidate buffers: packed (key | exact weight, 12
+  // bytes) or lane records (key + L residues).
+  const bool packed = packed_records(spec);
+  const int LK = packed ? kPackedRecords : L;
+  const size_t rb = record_bytes(LK);
+  const double R = packed ? 12.0 : 8.0 + 4.0 * L;  // algorithmic bytes per record
   Primes pr{};
   for (int l = 0; l < L; ++l) pr.p[l] = (uint32_t)spec.primes[l];
+  pr.lanes = (uint32_t)L;
   u128 fact;
   factorial(n, &fact);
 
@@ -569,6 +628,7 @@ int Engine::pipeline(const RunSpec& spec, Plan* plan, bool eager, RunOut* out, s
   g.dr_out = occupied_.diag_dr_out ? g.board_mask : 0u;
   g.dl_in = occupied_.diag_dl_in ? g.board_mask : 0u;
   g.dl_out = occupied_.diag_dl_out ? g.board_mask : 0u;
+  packed_widths(spec, &g.key_bits, &g.weight_bits);
   const uint64_t final_mask = g.board_mask;
 
   const uint32_t rbits = std::max<uint32_t>(1, std::min<uint32_t>(spec.radix_bits, kMaxDigitBits));
@@ -585,7 +645,7 @@ int Engine::pipeline(const RunSpec& spec, Plan* plan, bool eager, RunOut* out, s
   // One full wave of resident SCATTER workgroups (times merge_waves), so the
   // bucket ranges finish together instead of leaving a partly occupied tail.
   const uint64_t merge_grid = (uint64_t)cu_count_ *
-                              merge_resident_blocks(L, table_slots, std::min<uint32_t>(spec.expand_bits, kFineBits)) *
+                              merge_resident_blocks(LK, table_slots, std::min<uint32_t>(spec.expand_bits, kFineBits)) *
                               std::max<uint32_t>(1, std::min<uint32_t>(spec.merge_waves, 8));
   auto split_bits = [&](unsigned B, unsigned d1) {
     std::vector<unsigned> d;
@@ -625,19 +685,14 @@ int Engine::pipeline(const RunSpec& spec, Plan* plan, bool eager, RunOut* out, s
   uint64_t* ssum = nslice ? slice_sums_ : nullptr;
 
   const uint64_t S0 = spec.seeds.size();
-  const size_t seed_bytes = S0 * (8 + 4 * (size_t)L);
   if (eager) {
-    rc = ensure_buffer(&seed_, &seed_cap_, (seed_bytes + 7) / 8, err);
+    std::vector<uint64_t> host;
+    if (seed_image(spec, &host)) return kRcPackedOverflow;
+    rc = ensure_buffer(&seed_, &seed_cap_, host.size(), err);
     if (rc) return rc;
-    std::vector<uint64_t> host((seed_bytes + 7) / 8, 0);
-    uint64_t* keys = host.data();
-    uint32_t* res = (uint32_t*)(keys + S0);
-    for (uint64_t i = 0; i < S0; ++i) {
-      keys[i] = spec.seeds[i].key;
-      for (int l = 0; l < L; ++l) res[(size_t)l * S0 + i] = (uint32_t)(spec.seeds[i].weight % spec.primes[l]);
-    }
     HIP_TRY(hipStreamSynchronize(stream_));
-    HIP_TRY(hipMemcpy(seed_, host.data(), seed_bytes, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(seed_, host.data(), host.size() * 8, hipMemcpyHostToDevice));
+    out->record_bytes = (uint32_t)rb;
     out->layers.clear();
     out->peak_states = S0;
     out->total_candidates = out->total_matched = 0;
@@ -699,13 +754,14 @@ int Engine::pipeline(const RunSpec& spec, Plan* plan, bool eager, RunOut* out, s
       if (e) return e;
       uint64_t* spare = row_counters(0) + n * kNumCounters;
       LAUNCH("part_hist", 8.0 * Crow,
-             launch_part_hist(L, region_[cur], segs_[sidx], (uint32_t)nseg, chunks, 64 - used, dk, hist_, stream_));
+             launch_part_hist(LK, region_[cur], segs_[sidx], (uint32_t)nseg, chunks, 64 - used, dk, g.key_bits,
+                              hist_, stream_));
       LAUNCH("scan", 12.0 * cells,
              launch_scan(hist_, (uint32_t)(nseg << dk), chunks, 1, offsets_, segs_[sidx ^ 1], partial_, spare,
                          stream_));
       LAUNCH("part_scatter", 2.0 * R * Crow,
-             launch_part_scatter(L, region_[cur], region_[to], segs_[sidx], (uint32_t)nseg, chunks, 64 - used, dk,
-                                 offsets_, region_bytes_[to] / rb, spare, stream_));
+             launch_part_scatter(LK, region_[cur], region_[to], segs_[sidx], (uint32_t)nseg, chunks, 64 - used, dk,
+                                 g.key_bits, offsets_, region_bytes_[to] / rb, spare, stream_));
       cur = to;
       sidx ^= 1;
       nseg <<= dk;
@@ -741,7 +797,7 @@ int Engine::pipeline(const RunSpec& spec, Plan* plan, bool eager, RunOut* out, s
     LAUNCH("scan", 12.0 * tiles * D1,
            launch_scan(hist_, D1, (uint32_t)tiles, 1, offsets_, segs_[0], partial_, c0, stream_));
     LAUNCH("expand_scatter", R * (S0 + C),
-           launch_expand_scatter(L, skeys, sres, S0, S0, g, pr, d1, (uint32_t)tiles, per_tile, offsets_,
+           launch_expand_scatter(LK, skeys, sres, S0, S0, g, pr, d1, (uint32_t)tiles, per_tile, offsets_,
                                  region_[0], region_bytes_[0] / rb, row_counters(0) + n * kNumCounters,
                                  stream_));
     rc = partition_to_final(0, d1, pd, C);
@@ -769,7 +825,7 @@ int Engine::pipeline(const RunSpec& spec, Plan* plan, bool eager, RunOut* out, s
       rp.W = W;
       if (last) {
         LAUNCH("merge_last", R * C,
-               launch_merge_expand(kMergeLast, L, region_[cur], segs_[sidx], (uint32_t)nseg, W, g, pr, 0,
+               launch_merge_expand(kMergeLast, LK, region_[cur], segs_[sidx], (uint32_t)nseg, W, g, pr, 0,
                                    nullptr, nullptr, nullptr, 0, final_mask, ssum, nslice / L, cr, table_slots,
                                    stream_));
       } else {
@@ -783,7 +839,7 @@ int Engine::pipeline(const RunSpec& spec, Plan* plan, bool eager, RunOut* out, s
         if (rc) return rc;
         uint64_t* cn = row_counters(row + 1);
         LAUNCH("merge_count", 8.0 * C,
-               launch_merge_expand(kMergeCount, L, region_[cur], segs_[sidx], (uint32_t)nseg, W, g, pr,
+               launch_merge_expand(kMergeCount, LK, region_[cur], segs_[sidx], (uint32_t)nseg, W, g, pr,
                                    kFineBits, hist_, nullptr, nullptr, 0, 0, nullptr, 0, cr, table_slots, stream_));
         LAUNCH("scan", 12.0 * DF * W, launch_scan(hist_, DF, W, 1, offsets_, segs_[nsidx], partial_, cn, stream_));
         if (eager) {
@@ -813,7 +869,7 @@ int Engine::pipeline(const RunSpec& spec, Plan* plan, bool eager, RunOut* out, s
           rc = grow(to, C_next * rb);
           if (rc) return rc;
           LAUNCH("merge_scatter", R * (C + C_next),
-                 launch_merge_expand(kMergeScatter, L, region_[cur], segs_[sidx], (uint32_t)nseg, W, g, pr, d1,
+                 launch_merge_expand(kMergeScatter, LK, region_[cur], segs_[sidx], (uint32_t)nseg, W, g, pr, d1,
                                      nullptr, offsets_, region_[to], region_bytes_[to] / rb, final_mask, ssum,
                                      nslice / L, cr, table_slots, stream_));
           sidx = nsidx;
@@ -822,7 +878,7 @@ int Engine::pipeline(const RunSpec& spec, Plan* plan, bool eager, RunOut* out, s
         } else {
           // No successor survives: merge for the statistics only.
           LAUNCH("merge_last", R * C,
-                 launch_merge_expand(kMergeLast, L, region_[cur], segs_[sidx], (uint32_t)nseg, W, g, pr, 0,
+                 launch_merge_expand(kMergeLast, LK, region_[cur], segs_[sidx], (uint32_t)nseg, W, g, pr, 0,
                                      nullptr, nullptr, nullptr, 0, final_mask, ssum, nslice / L, cr, table_slots,
                                      stream_));
         }
@@ -830,6 +886,7 @@ int Engine::pipeline(const RunSpec& spec, Plan* plan, bool eager, RunOut* out, s
       if (eager) {
         HIP_TRY(hipMemcpyAsync(h_counters_, cr, sizeof(uint64_t) * kNumCounters, hipMemcpyDeviceToHost, stream_));
         SYNC_AND_RESOLVE();
+        if (h_counters_[kError] & 4) return kRcPackedOverflow;
         if (h_counters_[kError]) {
           *err = "LDS merge table overflow in row " + std::to_string(row + 1);
           return 4;
@@ -877,6 +934,7 @@ int Engine::pipeline(const RunSpec& spec, Plan* plan, bool eager, RunOut* out, s
   // spare counter row n.
   std::vector<uint64_t> spare(kNumCounters);
   HIP_TRY(hipMemcpy(spare.data(), row_counters(n), sizeof(uint64_t) * kNumCounters, hipMemcpyDeviceToHost));
+  if (spare[kError] & 4) return kRcPackedOverflow;
   if (spare[kError]) {
     *err = "candidate buffer capacity exceeded";
     return 4;
@@ -923,10 +981,13 @@ int Engine::finish(const RunSpec& spec, const Plan& plan, RunOut* out, std::stri
   factorial(n, &fact);
   std::vector<uint64_t> cnt((size_t)(n + 1) * kNumCounters);
   HIP_TRY(hipMemcpy(cnt.data(), counters_rows_, cnt.size() * 8, hipMemcpyDeviceToHost));
+  for (unsigned row = 0; row <= n; ++row)
+    if (cnt[(size_t)row * kNumCounters + kError] & 4) return kRcPackedOverflow;
   if (cnt[(size_t)n * kNumCounters + kError]) {
     *err = "candidate buffer capacity exceeded (stale size plan)";
     return 4;
   }
+  out->record_bytes = (uint32_t)record_bytes(packed_records(spec) ? kPackedRecords : L);
   out->layers.clear();
   out->peak_states = spec.seeds.size();
   out->total_candidates = out->total_matched = 0;
@@ -1027,6 +1088,8 @@ int Engine::materialized(const RunSpec& spec, RunOut* out, std::string* err) {
   }
   Primes pr{};
   for (int l = 0; l < L; ++l) pr.p[l] = (uint32_t)spec.primes[l];
+  pr.lanes = (uint32_t)L;
+  out->record_bytes = (uint32_t)record_bytes(L);
   u128 fact;
   factorial(n, &fact);
 
@@ -1180,14 +1243,14 @@ int Engine::materialized(const RunSpec& spec, RunOut* out, std::string* err) {
         if (!rc) rc = ensure_buffer(&partial_, &partial_cap_, cells / 4096 + 2, err);
         if (rc) return rc;
         LAUNCH("part_hist", 8.0 * C,
-               launch_part_hist(L, region_[cur], segs_[sidx], (uint32_t)nseg, chunks, 64 - used, dk, hist_,
+               launch_part_hist(L, region_[cur], segs_[sidx], (uint32_t)nseg, chunks, 64 - used, dk, 0, hist_,
                                 stream_));
         LAUNCH("scan", 12.0 * cells,
                launch_scan(hist_, (uint32_t)(nseg << dk), chunks, 1, offsets_, segs_[sidx ^ 1], partial_, spare,
                            stream_));
         LAUNCH("part_scatter", 2.0 * R * C,
                launch_part_scatter(L, region_[cur], region_[to], segs_[sidx], (uint32_t)nseg, chunks, 64 - used,
-                                   dk, offsets_, region_bytes_[to] / rb, spare, stream_));
+                                   dk, 0, offsets_, region_bytes_[to] / rb, spare, stream_));
         cur = to;
         sidx ^= 1;
         nseg <<= dk;

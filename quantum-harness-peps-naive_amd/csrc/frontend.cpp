@@ -61,6 +61,7 @@ void fill_result(unsigned n, const RunOut& ro, const std::vector<uint64_t>& prim
   out->tensor_entries_examined = abi(examined);
   out->tensor_entries_matched = abi(matched);
   out->crt_lanes = (uint32_t)primes.size();
+  out->record_bytes = ro.record_bytes;
   for (size_t l = 0; l < primes.size() && l < PEPS_MAX_LANES; ++l) {
     out->primes[l] = primes[l];
     out->residues[l] = l < ro.residues.size() ? ro.residues[l] : 0;
@@ -130,6 +131,13 @@ int resolve_options(unsigned n, const peps_options* o, Resolved* r, std::string*
     return PEPS_E_INVALID;
   }
   r->site_counters = o->site_counters;
+  const uint32_t fmt = o->record_format & 0xffu, wbits = (o->record_format >> 8) & 0xffu;
+  if (fmt > PEPS_RECORDS_PACKED || (o->record_format >> 16)) {
+    *err = "record_format must be 0 (auto), 1 (lane records) or 2 (packed), with a packed weight width in bits 8..15";
+    return PEPS_E_INVALID;
+  }
+  r->records = (int)fmt;
+  r->packed_weight_bits = wbits ? wbits : kPackedMaxWeightBits;
   return PEPS_OK;
 }
 
@@ -154,6 +162,8 @@ void apply_resolved(const Resolved& r, RunSpec* spec) {
   spec->merge_passes = r.merge_passes;
   spec->merge_waves = r.merge_waves;
   spec->budget_bytes = r.budget;
+  spec->records = r.records;
+  spec->packed_weight_bits = r.packed_weight_bits;
 }
 
 int run_engine(const Resolved& r, RunSpec& spec, RunOut* ro, std::string* err) {

@@ -48,6 +48,8 @@ struct Resolved {
   uint32_t merge_passes = 1;
   uint32_t merge_waves = 8;
   uint32_t site_counters = 0;
+  int records = 0;                  // RecordFormat
+  uint32_t packed_weight_bits = 52;
 };
 
 // Lane count: the reference plan (src/lib.rs:3859-3886) unless the caller

@@ -2,7 +2,13 @@
 //
 // Keys are the reference's packed boundary (columns | diag_dr << n |
 // diag_dl << 2n, src/lib.rs:357-363) in a u64, optionally with a slice id
-// above bit 3n.  Weights are u32 residues modulo the CRT primes.
+// above bit 3n.  A candidate record is either the key with one u32 residue
+// per CRT prime ("lane records", 16 or 24 bytes) or a packed 12-byte record
+// holding the key in its low key_bits and the exact integer weight above it
+// (lanes == kPackedRecords).  Packed weights must stay below 2^weight_bits;
+// a kernel that meets a larger one sets bit 2 of kError and the run is
+// repeated with lane records.  Either way the layer statistics and final
+// sums are reduced to residues modulo every CRT prime.
 //
 // Row pipeline (candidates of row r live in hash-radix buckets):
 //   expand (seed rows only): frontier -> candidates, level-1 buckets
@@ -37,12 +43,19 @@ struct RowGeom {
   uint32_t orbit_mask;   // legal-position restriction (left half on the orbit row)
   uint32_t centre_bit;   // the odd-N fixed column on the orbit row (weight 1), else 0
   uint32_t col_in, col_out, dr_in, dr_out, dl_in, dl_out;  // leg masks
+  uint32_t key_bits;     // packed records: 3n + slice-id bits (<= 63)
+  uint32_t weight_bits;  // packed records: weights are < 2^weight_bits (<= 52)
   uint64_t state_mask;   // (1 << 3n) - 1; bits above are a slice id
 };
 
 struct Primes {
   uint32_t p[kMaxLanes];
+  uint32_t lanes;        // CRT lanes of the statistics (1..kMaxLanes)
 };
+
+// `lanes` value of the launchers that selects packed 12-byte records.
+constexpr int kPackedRecords = 0;
+constexpr uint32_t kPackedMaxWeightBits = 52;  // bucket sums stay below 2^64
 
 // Device-side counters of one row (u64 each).
 enum Counter : int {
@@ -50,7 +63,8 @@ enum Counter : int {
   kOutStates = 1,    // merged states of this row
   kWeight0 = 2,      // .. kWeight0 + L - 1: residue sums of merged states (mod p per block)
   kFinal0 = 6,       // .. kFinal0 + L - 1: same restricted to columns == board mask
-  kError = 10,       // nonzero: an LDS merge table overflowed
+  kError = 10,       // bit 0: LDS merge table overflow; bit 1: capacity; bit 2: packed
+                     // weight overflow; bit 3: sitewise partial rows
   kMatchedSlot = 11, // legal positions before the row-0 orbit filter
   kSiteVisits = 12,  // sitewise: partial-row visits (one per partial per site)
   kSiteMatched = 13, // sitewise: C entries applied (= entries examined through the input index)
@@ -85,7 +99,8 @@ hipError_t launch_expand_hist(const uint64_t* keys, uint64_t S, const RowGeom& g
                               uint32_t tiles, uint64_t per_tile, uint32_t* hist, hipStream_t stream);
 // `cap` is the output capacity in records: a write at or beyond it is skipped
 // and sets bit 1 of counters[kError] (only a stale replayed size plan can
-// cause it).
+// cause it).  Packed records (lanes == kPackedRecords) read the seed weights
+// as exact u64 values at (const uint64_t*)res (res_stride unused).
 hipError_t launch_expand_scatter(int lanes, const uint64_t* keys, const uint32_t* res,
                                  uint64_t S, uint64_t res_stride, const RowGeom& g, const Primes& pr,
                                  uint32_t digit_bits, uint32_t tiles, uint64_t per_tile,
@@ -131,18 +146,20 @@ constexpr uint64_t kMaxPartChunks = 4096;
 uint32_t partition_chunks(uint64_t records, uint64_t nseg);
 // The partition is three launches: hist, launch_scan(hist, nseg << digit_bits,
 // chunks, 1, offsets, subsegs, partial, counters), scatter.
+// key_bits: the packed records' key width (ignored for lane records).
 hipError_t launch_part_hist(int lanes, const void* in, const uint64_t* segs, uint32_t nseg, uint32_t chunks,
-                            uint32_t shift, uint32_t digit_bits, uint32_t* hist, hipStream_t stream);
+                            uint32_t shift, uint32_t digit_bits, uint32_t key_bits, uint32_t* hist,
+                            hipStream_t stream);
 hipError_t launch_part_scatter(int lanes, const void* in, void* out, const uint64_t* segs, uint32_t nseg,
-                               uint32_t chunks, uint32_t shift, uint32_t digit_bits, const uint64_t* offsets,
-                               uint64_t cap, uint64_t* counters, hipStream_t stream);
+                               uint32_t chunks, uint32_t shift, uint32_t digit_bits, uint32_t key_bits,
+                               const uint64_t* offsets, uint64_t cap, uint64_t* counters, hipStream_t stream);
 
 // Fused merge of row r's buckets with the expansion into row r+1.
 // COUNT: hist[(1<<kFineBits)][W] of successor digits.  SCATTER: successors
 // written to out (capacity `cap` records) at offsets[(1<<digit_bits)][W].
-// LAST: no expansion.  EMIT: the merged states are written to `out` as a
-// frontier (keys[cap] then residues[l * cap + i]), positions counted in
-// counters[kEmitted].  Merged states with (key & final_mask) == final_mask
+// LAST: no expansion.  EMIT (lane records only): the merged states are
+// written to `out` as a frontier (keys[cap] then residues[l * cap + i]),
+// positions counted in counters[kEmitted].  Merged states with (key & final_mask) == final_mask
 // (all columns set) are summed into kFinal0 and, if slice_sums is given,
 // into slice_sums[slice * L + l] (slice = key bits above 3N).
 // table_slots selects the LDS table (2048 or 4096 slots); a bucket of up to

@@ -20,6 +20,8 @@
 // them.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "kernels.h"
 
 namespace peps {
@@ -37,6 +39,48 @@ struct alignas(8) Rec {
   uint64_t key;
   uint32_t r[L];
 };
+
+// Packed record: 96 bits = key | weight << key_bits, as three u32 words
+// (12 bytes, 4-byte aligned).  key_bits is in [3, 63] and the weight below
+// 2^(96 - key_bits), so every shift below is in range.
+template <>
+struct alignas(4) Rec<kPackedRecords> {
+  uint32_t w[3];
+};
+using Packed = Rec<kPackedRecords>;
+
+__device__ __forceinline__ uint64_t packed_low(const Packed& r) {
+  return (uint64_t)r.w[0] | ((uint64_t)r.w[1] << 32);
+}
+__device__ __forceinline__ uint64_t packed_key(const Packed& r, uint32_t kb) {
+  return packed_low(r) & ((1ull << kb) - 1);
+}
+__device__ __forceinline__ uint64_t packed_weight(const Packed& r, uint32_t kb) {
+  return (packed_low(r) >> kb) | ((uint64_t)r.w[2] << (64 - kb));
+}
+__device__ __forceinline__ Packed make_packed(uint64_t key, uint64_t weight, uint32_t kb) {
+  const uint64_t lo = key | (weight << kb);
+  Packed r;
+  r.w[0] = (uint32_t)lo;
+  r.w[1] = (uint32_t)(lo >> 32);
+  r.w[2] = (uint32_t)(weight >> (64 - kb));
+  return r;
+}
+
+// Key of either record kind (kb: packed key width).
+template <int L>
+__device__ __forceinline__ uint64_t rec_key(const Rec<L>& r, uint32_t kb) {
+  if constexpr (L == kPackedRecords)
+    return packed_key(r, kb);
+  else
+    return r.key;
+}
+
+// Residue of an exact 128-bit sum (hi:lo) modulo p < 2^32.
+__device__ __forceinline__ uint64_t mod_u128(uint64_t hi, uint64_t lo, uint32_t p) {
+  const uint64_t t64 = (~0ull % p + 1) % p;  // 2^64 mod p
+  return ((hi % p) * t64 % p + lo % p) % p;
+}
 
 // Workgroup barrier that waits for LDS operations only.  Outstanding global
 // loads keep flying; their consumers are guarded by the compiler's own
@@ -182,7 +226,7 @@ __device__ __forceinline__ void scan_digits(const uint32_t* counts, uint32_t* st
 template <int L, int DMAX>
 struct Stage {
   Rec<L> rec[kStage];
-  uint16_t bucket[kStage];
+  std::conditional_t<(DMAX <= 256), uint8_t, uint16_t> bucket[kStage];
   uint16_t rank[kStage];
   uint16_t order[kStage];
   uint32_t count[DMAX];
@@ -195,7 +239,7 @@ template <int L, int DMAX>
 __device__ __forceinline__ void stage_put(Stage<L, DMAX>& st, uint32_t slot, const Rec<L>& r,
                                           uint32_t b) {
   st.rec[slot] = r;
-  st.bucket[slot] = (uint16_t)b;
+  st.bucket[slot] = (std::remove_reference_t<decltype(st.bucket[0])>)b;
   st.rank[slot] = (uint16_t)atomicAdd(&st.count[b], 1u);
 }
 
@@ -297,13 +341,18 @@ __global__ __launch_bounds__(kBlock) void k_expand_scatter(
     uint64_t key = 0;
     uint32_t free = 0;
     Board bs{0, 0, 0};
-    uint32_t r[L];
+    uint32_t r[L > 0 ? L : 1];
+    uint64_t wt = 0;  // packed records: the exact seed weight
     if (i < hi) {
       key = keys[i];
       bs = unpack(key, g);
       free = legal_positions(bs, g);
+      if constexpr (L == kPackedRecords) {
+        wt = ((const uint64_t*)res)[i];
+      } else {
 #pragma unroll
-      for (int l = 0; l < L; ++l) r[l] = res[(uint64_t)l * res_stride + i];
+        for (int l = 0; l < L; ++l) r[l] = res[(uint64_t)l * res_stride + i];
+      }
     }
     const uint32_t cnt = (uint32_t)__popc(free);
     uint32_t total;
@@ -325,11 +374,18 @@ __global__ __launch_bounds__(kBlock) void k_expand_scatter(
           const uint32_t idx = pos + j;
           if (idx < win || idx >= wend) continue;
           Rec<L> rec;
-          rec.key = successor(key, bs, b, g);
+          const uint64_t sk = successor(key, bs, b, g);
           const bool twice = g.orbit_row && b != g.centre_bit;
+          if constexpr (L == kPackedRecords) {
+            const uint64_t w = twice ? 2 * wt : wt;  // seeds are below 2^52: no wrap
+            if (w >> g.weight_bits) atomicOr((unsigned long long*)&counters[kError], 4ull);
+            rec = make_packed(sk, w, g.key_bits);
+          } else {
+            rec.key = sk;
 #pragma unroll
-          for (int l = 0; l < L; ++l) rec.r[l] = twice ? addmod(r[l], r[l], pr.p[l]) : r[l];
-          stage_put<L, kExpandDigits>(st, staged + (idx - win), rec, top_digit(rec.key, digit_bits));
+            for (int l = 0; l < L; ++l) rec.r[l] = twice ? addmod(r[l], r[l], pr.p[l]) : r[l];
+          }
+          stage_put<L, kExpandDigits>(st, staged + (idx - win), rec, top_digit(sk, digit_bits));
         }
       }
       staged += wend - win;
@@ -619,7 +675,7 @@ template <int L, int PB>
 __global__ __launch_bounds__(kBlock) void k_part_hist(const Rec<L>* __restrict__ in,
                                                       const uint64_t* __restrict__ segs, uint32_t nseg,
                                                       uint32_t chunks, uint32_t shift, uint32_t digit_bits,
-                                                      uint32_t* __restrict__ hist) {
+                                                      uint32_t kb, uint32_t* __restrict__ hist) {
   __shared__ uint32_t cnt[1 << PB];
   const uint32_t D = 1u << digit_bits;
   const uint64_t dmask = D - 1;
@@ -635,7 +691,7 @@ __global__ __launch_bounds__(kBlock) void k_part_hist(const Rec<L>* __restrict__
 #pragma unroll
       for (int j = 0; j < kPerThread; ++j) {
         const uint64_t i = base + (uint64_t)j * kBlock + threadIdx.x;
-        k[j] = i < b ? in[i].key : kEmptyKey;
+        k[j] = i < b ? rec_key(in[i], kb) : kEmptyKey;
       }
 #pragma unroll
       for (int j = 0; j < kPerThread; ++j)
@@ -652,8 +708,8 @@ __global__ __launch_bounds__(kBlock) void k_part_scatter(const Rec<L>* __restric
                                                          Rec<L>* __restrict__ out,
                                                          const uint64_t* __restrict__ segs, uint32_t nseg,
                                                          uint32_t chunks, uint32_t shift, uint32_t digit_bits,
-                                                         const uint64_t* __restrict__ offsets, uint64_t cap,
-                                                         uint64_t* __restrict__ counters) {
+                                                         uint32_t kb, const uint64_t* __restrict__ offsets,
+                                                         uint64_t cap, uint64_t* __restrict__ counters) {
   __shared__ Stage<L, (1 << PB)> st;
   const uint32_t D = 1u << digit_bits;
   const uint64_t dmask = D - 1;
@@ -690,7 +746,7 @@ __global__ __launch_bounds__(kBlock) void k_part_scatter(const Rec<L>* __restric
       for (int j = 0; j < kPerThread; ++j) {
         const uint32_t slot = (uint32_t)j * kBlock + threadIdx.x;
         if (slot < nrec)
-          stage_put<L, (1 << PB)>(st, slot, cur[j], (uint32_t)((mix64(cur[j].key) >> shift) & dmask));
+          stage_put<L, (1 << PB)>(st, slot, cur[j], (uint32_t)((mix64(rec_key(cur[j], kb)) >> shift) & dmask));
       }
       stage_flush<L, (1 << PB)>(st, nrec, D, out, cap, counters);
     }
@@ -726,11 +782,15 @@ __global__ __launch_bounds__(BT, (BT == kScatterBlock && L <= 2 && T == 2048 && 
   constexpr uint32_t kCap = (T * 5) / 8;             // records per pass
   constexpr int KB = (int)((kCap + BT - 1) / BT);  // records per thread per batch
   constexpr uint32_t kSpan = (uint32_t)KB * BT;  // records per batch (>= kCap)
-  constexpr int LL = MODE == kMergeCount ? 1 : L;   // COUNT needs no residues
+  constexpr bool P = L == kPackedRecords;           // packed records: exact weights
+  constexpr int NL = P ? 1 : L;                     // weight words per record
+  constexpr int LL = (MODE == kMergeCount || P) ? 1 : L;  // residue rows of the table
   constexpr bool kExpand = MODE == kMergeCount || MODE == kMergeScatter;
   constexpr uint32_t DM = kExpand ? (1u << DB) : 1;
+  static_assert(!(P && MODE == kMergeEmit), "EMIT writes lane records only");
   __shared__ uint64_t tkey[T];
-  __shared__ uint32_t tres[LL][T];
+  __shared__ uint32_t tres[LL][T];                            // lane records
+  __shared__ uint64_t tw[(P && MODE != kMergeCount) ? T : 1];  // packed: exact weights
   __shared__ uint16_t ulist[T];
   __shared__ uint32_t lcount[DM];
   __shared__ uint64_t gpos[MODE == kMergeScatter ? DM : 1];
@@ -738,8 +798,9 @@ __global__ __launch_bounds__(BT, (BT == kScatterBlock && L <= 2 && T == 2048 && 
   __shared__ uint32_t nuniq[2];
   // Last row: per-slice sums gathered in LDS, flushed with one global atomic
   // per (slice, lane) and workgroup.
+  // Packed records keep one exact sum per slice (copied into every lane).
   constexpr uint32_t kLdsSlices = MODE == kMergeLast ? 1024u : 1u;
-  __shared__ uint64_t ssl[kLdsSlices * L];
+  __shared__ uint64_t ssl[kLdsSlices * NL];
   const bool lds_slices = MODE == kMergeLast && slice_sums && nslices <= kLdsSlices;
   const uint32_t D = kExpand ? (1u << digit_bits) : 1u;
   const uint32_t W = gridDim.x;
@@ -754,13 +815,15 @@ __global__ __launch_bounds__(BT, (BT == kScatterBlock && L <= 2 && T == 2048 && 
   }
   if (threadIdx.x < 2) nuniq[threadIdx.x] = 0;
   if (lds_slices)
-    for (uint32_t i = threadIdx.x; i < nslices * L; i += BT) ssl[i] = 0;
+    for (uint32_t i = threadIdx.x; i < nslices * NL; i += BT) ssl[i] = 0;
   lds_barrier();
 
   uint64_t uniq = 0;
-  uint64_t wsum[L], fsum[L];
+  // Lane records: per-thread residue sums.  Packed: exact 128-bit sums
+  // (index 0 low, 1 high) reduced to residues at the end.
+  uint64_t wsum[P ? 2 : L], fsum[P ? 2 : L];
 #pragma unroll
-  for (int l = 0; l < L; ++l) wsum[l] = fsum[l] = 0;
+  for (int l = 0; l < (P ? 2 : L); ++l) wsum[l] = fsum[l] = 0;
   uint32_t overflow = 0;
   uint32_t parity = 0;
 
@@ -773,10 +836,14 @@ __global__ __launch_bounds__(BT, (BT == kScatterBlock && L <= 2 && T == 2048 && 
     for (int k = 0; k < KB; ++k) {
       const uint64_t i = base + (uint64_t)k * BT + threadIdx.x;
       if (i < hi) {
-        if (MODE == kMergeCount)
+        if constexpr (MODE == kMergeCount && P) {
+          buf[k].w[0] = in[i].w[0];
+          buf[k].w[1] = in[i].w[1];
+        } else if constexpr (MODE == kMergeCount) {
           buf[k].key = in[i].key;
-        else
+        } else {
           buf[k] = in[i];
+        }
       }
     }
   };
@@ -808,7 +875,7 @@ __global__ __launch_bounds__(BT, (BT == kScatterBlock && L <= 2 && T == 2048 && 
           sl[k] = 0;
           const uint64_t i = base + (uint64_t)k * BT + threadIdx.x;
           if (i >= hi) continue;
-          const uint64_t key = buf[k].key;
+          const uint64_t key = rec_key(buf[k], g.key_bits);
           const uint64_t h = mix64(key);
           if (passes > 1 && (uint32_t)((((h >> 12) & 0xffffffull) * passes) >> 24) != pass) continue;
           uint32_t slot = (uint32_t)h & (T - 1);
@@ -849,27 +916,37 @@ __global__ __launch_bounds__(BT, (BT == kScatterBlock && L <= 2 && T == 2048 && 
         }
         if (MODE != kMergeCount) {
           lds_barrier();
-          // ---- 2. owners store their residues
+          // ---- 2. owners store their residues (packed: exact weights)
 #pragma unroll
           for (int k = 0; k < KB; ++k)
             if ((own >> k) & 1u) {
+              if constexpr (P) {
+                tw[sl[k]] = packed_weight(buf[k], g.key_bits);
+              } else {
 #pragma unroll
-              for (int l = 0; l < L; ++l) tres[l < LL ? l : 0][sl[k]] = buf[k].r[l];
+                for (int l = 0; l < L; ++l) tres[l < LL ? l : 0][sl[k]] = buf[k].r[l];
+              }
             }
           lds_barrier();
-          // ---- 3. duplicates add modulo each prime
+          // ---- 3. duplicates add modulo each prime (packed: plain 64-bit
+          // adds; a bucket's sum of weights below 2^52 stays below 2^64)
           if (dup) {
 #pragma unroll
             for (int k = 0; k < KB; ++k)
               if ((dup >> k) & 1u) {
+                if constexpr (P) {
+                  atomicAdd((unsigned long long*)&tw[sl[k]],
+                            (unsigned long long)packed_weight(buf[k], g.key_bits));
+                } else {
 #pragma unroll
-                for (int l = 0; l < L; ++l) {
-                  uint32_t* a = &tres[l < LL ? l : 0][sl[k]];
-                  uint32_t old = *a, assumed;
-                  do {
-                    assumed = old;
-                    old = atomicCAS(a, assumed, addmod(assumed, buf[k].r[l], pr.p[l]));
-                  } while (old != assumed);
+                  for (int l = 0; l < L; ++l) {
+                    uint32_t* a = &tres[l < LL ? l : 0][sl[k]];
+                    uint32_t old = *a, assumed;
+                    do {
+                      assumed = old;
+                      old = atomicCAS(a, assumed, addmod(assumed, buf[k].r[l], pr.p[l]));
+                    } while (old != assumed);
+                  }
                 }
               }
           }
@@ -885,31 +962,51 @@ __global__ __launch_bounds__(BT, (BT == kScatterBlock && L <= 2 && T == 2048 && 
         const uint64_t key = tkey[slot];
         tkey[slot] = kEmptyKey;
         uniq += 1;
-        uint32_t v[L];
+        uint32_t v[NL];
+        uint64_t wv = 0;  // packed: the merged exact weight
         if (MODE != kMergeCount) {
+          if constexpr (P) {
+            wv = tw[slot];
+            if (wv >> g.weight_bits) overflow |= 4u;
+            wsum[0] += wv;
+            wsum[1] += wsum[0] < wv;
+          } else {
 #pragma unroll
-          for (int l = 0; l < L; ++l) {
-            v[l] = tres[l < LL ? l : 0][slot];
-            wsum[l] += v[l];
+            for (int l = 0; l < L; ++l) {
+              v[l] = tres[l < LL ? l : 0][slot];
+              wsum[l] += v[l];
+            }
           }
           // Complete boundaries (every column set) end here: the final v1/v2
           // contraction sums them, per slice when slices are tracked.
           if (final_mask && (key & final_mask) == final_mask) {
-#pragma unroll
-            for (int l = 0; l < L; ++l) fsum[l] += v[l];
-            if (slice_sums) {
-              const uint64_t sl = (key & ~g.state_mask) >> (3 * g.n);
-#pragma unroll
-              for (int l = 0; l < L; ++l) {
+            const uint64_t sl = (key & ~g.state_mask) >> (3 * g.n);
+            if constexpr (P) {
+              fsum[0] += wv;
+              fsum[1] += fsum[0] < wv;
+              if (slice_sums) {
                 if (lds_slices)
-                  atomicAdd((unsigned long long*)&ssl[sl * L + l], (unsigned long long)v[l]);
+                  atomicAdd((unsigned long long*)&ssl[sl], (unsigned long long)wv);
                 else
-                  atomicAdd((unsigned long long*)&slice_sums[sl * L + l], (unsigned long long)v[l]);
+                  for (uint32_t l = 0; l < pr.lanes; ++l)
+                    atomicAdd((unsigned long long*)&slice_sums[sl * pr.lanes + l], (unsigned long long)wv);
+              }
+            } else {
+#pragma unroll
+              for (int l = 0; l < L; ++l) fsum[l] += v[l];
+              if (slice_sums) {
+#pragma unroll
+                for (int l = 0; l < L; ++l) {
+                  if (lds_slices)
+                    atomicAdd((unsigned long long*)&ssl[sl * L + l], (unsigned long long)v[l]);
+                  else
+                    atomicAdd((unsigned long long*)&slice_sums[sl * L + l], (unsigned long long)v[l]);
+                }
               }
             }
           }
         }
-        if (MODE == kMergeEmit) {
+        if constexpr (MODE == kMergeEmit) {
           // One global atomic per wave: lanes take consecutive positions.
           uint64_t* ek = (uint64_t*)out;
           uint32_t* er = (uint32_t*)(ek + cap);
@@ -942,9 +1039,13 @@ __global__ __launch_bounds__(BT, (BT == kScatterBlock && L <= 2 && T == 2048 && 
             } else {
               const uint32_t rank = atomicAdd(&lcount[d], 1u);
               Rec<L> rec;
-              rec.key = sk;
+              if constexpr (P) {
+                rec = make_packed(sk, wv, g.key_bits);
+              } else {
+                rec.key = sk;
 #pragma unroll
-              for (int l = 0; l < L; ++l) rec.r[l] = v[l];
+                for (int l = 0; l < L; ++l) rec.r[l] = v[l];
+              }
               const uint64_t pos = gpos[d] + rank;
               if (pos < cap)
                 out[pos] = rec;
@@ -968,11 +1069,19 @@ __global__ __launch_bounds__(BT, (BT == kScatterBlock && L <= 2 && T == 2048 && 
   } else {
     const uint64_t u = block_sum_u64<BT / 64>(uniq, scratch);
     if (threadIdx.x == 0 && u) atomicAdd((unsigned long long*)&counters[kOutStates], (unsigned long long)u);
+    if constexpr (P) {
+      // pr.lanes is uniform, so every thread takes the same block sums.
+      for (uint32_t l = 0; l < pr.lanes; ++l) {
+        const uint64_t w = block_sum_u64<BT / 64>(mod_u128(wsum[1], wsum[0], pr.p[l]), scratch);
+        if (threadIdx.x == 0 && w) atomicAdd((unsigned long long*)&counters[kWeight0 + l], (unsigned long long)(w % pr.p[l]));
+        const uint64_t f = block_sum_u64<BT / 64>(mod_u128(fsum[1], fsum[0], pr.p[l]), scratch);
+        if (threadIdx.x == 0 && f) atomicAdd((unsigned long long*)&counters[kFinal0 + l], (unsigned long long)(f % pr.p[l]));
+      }
+    } else {
 #pragma unroll
-    for (int l = 0; l < L; ++l) {
-      const uint64_t w = block_sum_u64<BT / 64>(wsum[l] % pr.p[l], scratch);
-      if (threadIdx.x == 0 && w) atomicAdd((unsigned long long*)&counters[kWeight0 + l], (unsigned long long)(w % pr.p[l]));
-      {
+      for (int l = 0; l < L; ++l) {
+        const uint64_t w = block_sum_u64<BT / 64>(wsum[l] % pr.p[l], scratch);
+        if (threadIdx.x == 0 && w) atomicAdd((unsigned long long*)&counters[kWeight0 + l], (unsigned long long)(w % pr.p[l]));
         const uint64_t f = block_sum_u64<BT / 64>(fsum[l] % pr.p[l], scratch);
         if (threadIdx.x == 0 && f) atomicAdd((unsigned long long*)&counters[kFinal0 + l], (unsigned long long)(f % pr.p[l]));
       }
@@ -980,8 +1089,15 @@ __global__ __launch_bounds__(BT, (BT == kScatterBlock && L <= 2 && T == 2048 && 
   }
   if (lds_slices) {
     lds_barrier();
-    for (uint32_t i = threadIdx.x; i < nslices * L; i += BT)
-      if (ssl[i]) atomicAdd((unsigned long long*)&slice_sums[i], (unsigned long long)ssl[i]);
+    if constexpr (P) {
+      for (uint32_t i = threadIdx.x; i < nslices; i += BT)
+        if (ssl[i])
+          for (uint32_t l = 0; l < pr.lanes; ++l)
+            atomicAdd((unsigned long long*)&slice_sums[i * pr.lanes + l], (unsigned long long)ssl[i]);
+    } else {
+      for (uint32_t i = threadIdx.x; i < nslices * L; i += BT)
+        if (ssl[i]) atomicAdd((unsigned long long*)&slice_sums[i], (unsigned long long)ssl[i]);
+    }
   }
   if (overflow) atomicOr((unsigned long long*)&counters[kError], (unsigned long long)overflow);
 }
@@ -995,7 +1111,9 @@ uint32_t grid_for(uint64_t items, uint32_t per_block, uint32_t cap) {
 }  // namespace
 
 size_t record_bytes(int lanes) {
+  static_assert(sizeof(Packed) == 12, "packed records are 12 bytes");
   switch (lanes) {
+    case kPackedRecords: return sizeof(Packed);
     case 1: return sizeof(Rec<1>);
     case 2: return sizeof(Rec<2>);
     case 3: return sizeof(Rec<3>);
@@ -1028,6 +1146,7 @@ hipError_t launch_expand_scatter(int lanes, const uint64_t* keys, const uint32_t
   if (digit_bits > kFineBits || tiles == 0) return hipErrorInvalidValue;
   dim3 grid(tiles), block(kBlock);
   switch (lanes) {
+    case kPackedRecords: hipLaunchKernelGGL(k_expand_scatter<kPackedRecords>, grid, block, 0, stream, keys, res, S, res_stride, g, pr, digit_bits, per_tile, offsets, (Packed*)out, cap, counters); break;
     case 1: hipLaunchKernelGGL(k_expand_scatter<1>, grid, block, 0, stream, keys, res, S, res_stride, g, pr, digit_bits, per_tile, offsets, (Rec<1>*)out, cap, counters); break;
     case 2: hipLaunchKernelGGL(k_expand_scatter<2>, grid, block, 0, stream, keys, res, S, res_stride, g, pr, digit_bits, per_tile, offsets, (Rec<2>*)out, cap, counters); break;
     case 3: hipLaunchKernelGGL(k_expand_scatter<3>, grid, block, 0, stream, keys, res, S, res_stride, g, pr, digit_bits, per_tile, offsets, (Rec<3>*)out, cap, counters); break;
@@ -1063,7 +1182,8 @@ uint32_t partition_chunks(uint64_t records, uint64_t nseg) {
 }
 
 hipError_t launch_part_hist(int lanes, const void* in, const uint64_t* segs, uint32_t nseg, uint32_t chunks,
-                            uint32_t shift, uint32_t digit_bits, uint32_t* hist, hipStream_t stream) {
+                            uint32_t shift, uint32_t digit_bits, uint32_t key_bits, uint32_t* hist,
+                            hipStream_t stream) {
   if (digit_bits == 0 || digit_bits > kMaxDigitBits || nseg == 0 || chunks == 0) return hipErrorInvalidValue;
   if (((uint64_t)nseg << digit_bits) > 0xffffffffull) return hipErrorInvalidValue;
   const uint64_t total = (uint64_t)nseg * chunks;
@@ -1072,12 +1192,13 @@ hipError_t launch_part_hist(int lanes, const void* in, const uint64_t* segs, uin
 #define PEPS_PH(LN)                                                                                           \
   if (narrow)                                                                                                 \
     hipLaunchKernelGGL((k_part_hist<LN, kNarrowDigitBits>), grid, block, 0, stream, (const Rec<LN>*)in, segs, \
-                       nseg, chunks, shift, digit_bits, hist);                                               \
+                       nseg, chunks, shift, digit_bits, key_bits, hist);                                     \
   else                                                                                                        \
     hipLaunchKernelGGL((k_part_hist<LN, kMaxDigitBits>), grid, block, 0, stream, (const Rec<LN>*)in, segs,    \
-                       nseg, chunks, shift, digit_bits, hist);                                               \
+                       nseg, chunks, shift, digit_bits, key_bits, hist);                                     \
   break
   switch (lanes) {
+    case kPackedRecords: PEPS_PH(kPackedRecords);
     case 1: PEPS_PH(1);
     case 2: PEPS_PH(2);
     case 3: PEPS_PH(3);
@@ -1089,8 +1210,8 @@ hipError_t launch_part_hist(int lanes, const void* in, const uint64_t* segs, uin
 }
 
 hipError_t launch_part_scatter(int lanes, const void* in, void* out, const uint64_t* segs, uint32_t nseg,
-                               uint32_t chunks, uint32_t shift, uint32_t digit_bits, const uint64_t* offsets,
-                               uint64_t cap, uint64_t* counters, hipStream_t stream) {
+                               uint32_t chunks, uint32_t shift, uint32_t digit_bits, uint32_t key_bits,
+                               const uint64_t* offsets, uint64_t cap, uint64_t* counters, hipStream_t stream) {
   if (digit_bits == 0 || digit_bits > kMaxDigitBits || nseg == 0 || chunks == 0) return hipErrorInvalidValue;
   const uint64_t total = (uint64_t)nseg * chunks;
   dim3 grid(total < 65536 ? (uint32_t)total : 65536u), block(kBlock);
@@ -1098,12 +1219,13 @@ hipError_t launch_part_scatter(int lanes, const void* in, void* out, const uint6
 #define PEPS_PS(LN)                                                                                            \
   if (narrow)                                                                                                  \
     hipLaunchKernelGGL((k_part_scatter<LN, kNarrowDigitBits>), grid, block, 0, stream, (const Rec<LN>*)in,     \
-                       (Rec<LN>*)out, segs, nseg, chunks, shift, digit_bits, offsets, cap, counters);          \
+                       (Rec<LN>*)out, segs, nseg, chunks, shift, digit_bits, key_bits, offsets, cap, counters); \
   else                                                                                                         \
     hipLaunchKernelGGL((k_part_scatter<LN, kMaxDigitBits>), grid, block, 0, stream, (const Rec<LN>*)in,        \
-                       (Rec<LN>*)out, segs, nseg, chunks, shift, digit_bits, offsets, cap, counters);          \
+                       (Rec<LN>*)out, segs, nseg, chunks, shift, digit_bits, key_bits, offsets, cap, counters); \
   break
   switch (lanes) {
+    case kPackedRecords: PEPS_PS(kPackedRecords);
     case 1: PEPS_PS(1);
     case 2: PEPS_PS(2);
     case 3: PEPS_PS(3);
@@ -1165,7 +1287,12 @@ static hipError_t launch_me(int mode, dim3 grid, hipStream_t stream, const void*
         hipLaunchKernelGGL((k_merge_expand<L, kMergeScatter, T, kFineBits, kScatterBlock>), grid, dim3(kScatterBlock), 0, stream, i, segs, nseg, per, g, pr, digit_bits, hist, offsets, o, cap, final_mask, slice_sums, nslices, counters);
       break;
     case kMergeLast: hipLaunchKernelGGL((k_merge_expand<L, kMergeLast, T>), grid, dim3(kBlock), 0, stream, i, segs, nseg, per, g, pr, digit_bits, hist, offsets, o, cap, final_mask, slice_sums, nslices, counters); break;
-    case kMergeEmit: hipLaunchKernelGGL((k_merge_expand<L, kMergeEmit, T>), grid, dim3(kBlock), 0, stream, i, segs, nseg, per, g, pr, digit_bits, hist, offsets, o, cap, final_mask, slice_sums, nslices, counters); break;
+    case kMergeEmit:
+      if constexpr (L == kPackedRecords)
+        return hipErrorInvalidValue;
+      else
+        hipLaunchKernelGGL((k_merge_expand<L, kMergeEmit, T>), grid, dim3(kBlock), 0, stream, i, segs, nseg, per, g, pr, digit_bits, hist, offsets, o, cap, final_mask, slice_sums, nslices, counters);
+      break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
@@ -1181,6 +1308,7 @@ uint32_t merge_resident_blocks(int lanes, uint32_t table_slots, uint32_t digit_b
                                      : (const void*)&k_merge_expand<LN, kMergeScatter, TS, kFineBits, kScatterBlock>
   if (table_slots == 2048) {
     switch (lanes) {
+      case kPackedRecords: PEPS_KF(kPackedRecords, 2048); break;
       case 1: PEPS_KF(1, 2048); break;
       case 2: PEPS_KF(2, 2048); break;
       case 3: PEPS_KF(3, 2048); break;
@@ -1188,6 +1316,7 @@ uint32_t merge_resident_blocks(int lanes, uint32_t table_slots, uint32_t digit_b
     }
   } else {
     switch (lanes) {
+      case kPackedRecords: PEPS_KF(kPackedRecords, 4096); break;
       case 1: PEPS_KF(1, 4096); break;
       case 2: PEPS_KF(2, 4096); break;
       case 3: PEPS_KF(3, 4096); break;
@@ -1211,6 +1340,7 @@ hipError_t launch_merge_expand(int mode, int lanes, const void* in, const uint64
 #define PEPS_ME(LN, TS) return launch_me<LN, TS>(mode, grid, stream, in, segs, nseg, per, g, pr, digit_bits, hist, offsets, out, cap, final_mask, slice_sums, nslices, counters)
   if (table_slots == 2048) {
     switch (lanes) {
+      case kPackedRecords: PEPS_ME(kPackedRecords, 2048);
       case 1: PEPS_ME(1, 2048);
       case 2: PEPS_ME(2, 2048);
       case 3: PEPS_ME(3, 2048);
@@ -1220,6 +1350,7 @@ hipError_t launch_merge_expand(int mode, int lanes, const void* in, const uint64
   }
   if (table_slots == 4096) {
     switch (lanes) {
+      case kPackedRecords: PEPS_ME(kPackedRecords, 4096);
       case 1: PEPS_ME(1, 4096);
       case 2: PEPS_ME(2, 4096);
       case 3: PEPS_ME(3, 4096);

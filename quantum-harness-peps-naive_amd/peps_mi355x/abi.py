@@ -29,6 +29,10 @@ PEPS_ROW_OPERATOR_SITEWISE = 1
 PEPS_SYMMETRY_NONE = 0
 PEPS_SYMMETRY_TOP_ROW_ORBITS = 1
 
+PEPS_RECORDS_AUTO = 0
+PEPS_RECORDS_LANES = 1
+PEPS_RECORDS_PACKED = 2
+
 
 # peps_progress_fn: (user, done, total)
 PROGRESS_FN = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint64)
@@ -86,6 +90,7 @@ class ContractionResult(ctypes.Structure):
         ("layer_count", ctypes.c_uint32),
         ("layers", LayerMetric * PEPS_MAX_N),
         ("crt_lanes", ctypes.c_uint32),
+        ("record_bytes", ctypes.c_uint32),
         ("primes", ctypes.c_uint64 * PEPS_MAX_LANES),
         ("residues", ctypes.c_uint64 * PEPS_MAX_LANES),
         ("device_peak_bytes", ctypes.c_uint64),
@@ -125,7 +130,7 @@ class Options(ctypes.Structure):
         ("merge_passes", ctypes.c_uint32),
         ("merge_waves", ctypes.c_uint32),
         ("site_counters", ctypes.c_uint32),
-        ("reserved", ctypes.c_uint32 * 1),
+        ("record_format", ctypes.c_uint32),
     ]
 
 

@@ -62,6 +62,8 @@ class PepsOptions:
     merge_passes: int = 0       # LDS merge passes per final bucket (0: default)
     merge_waves: int = 0        # merge grid in waves of resident workgroups (0: default)
     site_counters: int = 0      # sitewise tensor counters: 0 indexed, 1 linear scan of C
+    record_format: int = 0      # PEPS_RECORDS_*: 0 packed (lane fallback), 1 lane records, 2 packed only
+    packed_weight_bits: int = 0  # nonzero: narrower packed weight field (exercises the fallback)
 
     def to_c(self) -> Options:
         o = Options()
@@ -77,6 +79,7 @@ class PepsOptions:
         o.merge_passes = self.merge_passes
         o.merge_waves = self.merge_waves
         o.site_counters = self.site_counters
+        o.record_format = self.record_format | (self.packed_weight_bits << 8)
         return o
 
 

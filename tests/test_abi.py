@@ -119,3 +119,29 @@ def test_cli_grammar_and_errors():
     assert r.stdout.rstrip().endswith(
         "peak_states=1 tensor_entries_examined=0 tensor_entries_matched=0 "
         "row_operator_candidates=0 row_operator_matched=0 verified=true")
+
+
+def test_ctypes_layouts_match_the_c_header(tmp_path):
+    """Every field offset of the ctypes mirrors equals the C compiler's
+    offsetof for the same struct in include/peps_mi355x.h."""
+    from peps_mi355x.abi import ContractionResult, LayerMetric, Options
+    structs = {"peps_options": Options, "peps_contraction_result": ContractionResult,
+               "peps_layer_metric": LayerMetric}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "peps_mi355x.h"', "int main(void) {"]
+    for cname, cls in structs.items():
+        lines.append(f'  printf("{cname} sizeof %zu\\n", sizeof({cname}));')
+        for fname, _ in cls._fields_:
+            lines.append(f'  printf("{cname} {fname} %zu\\n", offsetof({cname}, {fname}));')
+    lines.append("  return 0;\n}")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c11", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    got = {}
+    for line in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.splitlines():
+        cname, fname, v = line.split()
+        got[(cname, fname)] = int(v)
+    for cname, cls in structs.items():
+        assert got[(cname, "sizeof")] == ctypes.sizeof(cls), cname
+        for fname, _ in cls._fields_:
+            assert got[(cname, fname)] == getattr(cls, fname).offset, (cname, fname)
