@@ -164,6 +164,7 @@ int peps_count_batched(uint32_t n, uint32_t split_depth, const peps_options* opt
     total.kernel_s += ro.kernel_s;
     total.alloc_s += ro.alloc_s;
     total.device_peak_bytes = std::max(total.device_peak_bytes, ro.device_peak_bytes);
+    total.record_bytes = std::max(total.record_bytes, ro.record_bytes);  // widest format any group used
     if (layers.empty()) {
       layers = ro.layers;
     } else {
@@ -213,7 +214,7 @@ int peps_count_batched(uint32_t n, uint32_t split_depth, const peps_options* opt
   while (next < tasks.size()) {
     // Admit tasks while their projected candidates fit 80% of a buffer.
     const double ratio = proj.ratio(n, split_depth);
-    const double cap = 0.8 * (double)eng->candidate_capacity((int)r.primes.size());
+    const double cap = 0.8 * (double)eng->candidate_capacity((int)r.primes.size(), r.records);
     const size_t g0 = next;
     size_t g1 = next + 1;
     double load = proxy[g0] * ratio;

@@ -140,8 +140,9 @@ class Engine {
                std::string* err);
   std::mutex& mutex() { return mu_; }
   int device() const { return device_; }
-  // Records one candidate buffer can hold (0 before the first run sets the budget).
-  uint64_t candidate_capacity(int lanes) const;
+  // Records one candidate buffer can hold (0 before the first run sets the
+  // budget), in the record format `records` selects.
+  uint64_t candidate_capacity(int lanes, int records) const;
   // Kernel timing with HIP events on the engine stream (off by default).
   void set_profiling(bool on);
   std::vector<std::pair<std::string, KernelStat>> kernel_stats() const;

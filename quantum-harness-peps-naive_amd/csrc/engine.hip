@@ -140,8 +140,8 @@ int Engine::ensure_region(int idx, size_t bytes, std::string* err) {
   return 0;
 }
 
-uint64_t Engine::candidate_capacity(int lanes) const {
-  return budget_ ? (budget_ / 2) / record_bytes(lanes) : 0;
+uint64_t Engine::candidate_capacity(int lanes, int records) const {
+  return budget_ ? (budget_ / 2) / record_bytes(records == kRecordsLanes ? lanes : kPackedRecords) : 0;
 }
 
 template <typename T>
