@@ -703,14 +703,35 @@ __global__ __launch_bounds__(kBlock) void k_part_hist(const Rec<L>* __restrict__
   }
 }
 
+// Per stage of PS records: every thread takes its records' digits and ranks
+// (one LDS atomic each, the rank kept in a register), the digit counts are
+// scanned, the records are placed in LDS already sorted by digit, and the
+// workgroup writes them out linearly -- consecutive threads store consecutive
+// addresses of one digit's run.
+template <int L>
+constexpr int part_stage() { return L == kPackedRecords ? 2560 : kStage; }
+
 template <int L, int PB>
-__global__ __launch_bounds__(kBlock) void k_part_scatter(const Rec<L>* __restrict__ in,
+struct PartStage {
+  Rec<L> rec[part_stage<L>()];
+  std::conditional_t<(PB <= 8), uint8_t, uint16_t> digit[part_stage<L>()];
+  uint32_t count[1 << PB];
+  uint32_t start[1 << PB];
+  uint64_t gpos[1 << PB];
+  uint32_t wave_tot[kWaves];
+};
+
+template <int L, int PB>
+__global__ __launch_bounds__(kBlock, (L == kPackedRecords && PB <= 8 ? 4 : 1)) void k_part_scatter(const Rec<L>* __restrict__ in,
                                                          Rec<L>* __restrict__ out,
                                                          const uint64_t* __restrict__ segs, uint32_t nseg,
                                                          uint32_t chunks, uint32_t shift, uint32_t digit_bits,
                                                          uint32_t kb, const uint64_t* __restrict__ offsets,
                                                          uint64_t cap, uint64_t* __restrict__ counters) {
-  __shared__ Stage<L, (1 << PB)> st;
+  constexpr int PS = part_stage<L>();
+  constexpr int PT = PS / kBlock;
+  static_assert(PS % kBlock == 0, "stage is whole rounds of the workgroup");
+  __shared__ PartStage<L, PB> st;
   const uint32_t D = 1u << digit_bits;
   const uint64_t dmask = D - 1;
   const uint64_t total = (uint64_t)nseg * chunks;
@@ -723,32 +744,62 @@ __global__ __launch_bounds__(kBlock) void k_part_scatter(const Rec<L>* __restric
       st.gpos[d] = offsets[((uint64_t)s * D + d) * chunks + p];
     }
     lds_barrier();
-    Rec<L> nx[kPerThread];
+    Rec<L> nx[PT];
 #pragma unroll
-    for (int j = 0; j < kPerThread; ++j) {
+    for (int j = 0; j < PT; ++j) {
       const uint64_t i = a + (uint64_t)j * kBlock + threadIdx.x;
       if (i < b) nx[j] = in[i];
     }
-    for (uint64_t base = a; base < b; base += kStage) {
-      const uint32_t nrec = (b - base < (uint64_t)kStage) ? (uint32_t)(b - base) : (uint32_t)kStage;
-      Rec<L> cur[kPerThread];
+    for (uint64_t base = a; base < b; base += PS) {
+      const uint32_t nrec = (b - base < (uint64_t)PS) ? (uint32_t)(b - base) : (uint32_t)PS;
+      Rec<L> cur[PT];
 #pragma unroll
-      for (int j = 0; j < kPerThread; ++j) cur[j] = nx[j];
-      const uint64_t nb = base + kStage;
+      for (int j = 0; j < PT; ++j) cur[j] = nx[j];
+      const uint64_t nb = base + PS;
       if (nb < b) {
 #pragma unroll
-        for (int j = 0; j < kPerThread; ++j) {
+        for (int j = 0; j < PT; ++j) {
           const uint64_t i = nb + (uint64_t)j * kBlock + threadIdx.x;
           if (i < b) nx[j] = in[i];
         }
       }
+      // 1. digit and rank of every staged record
+      uint32_t dg[PT], rk[PT];
 #pragma unroll
-      for (int j = 0; j < kPerThread; ++j) {
+      for (int j = 0; j < PT; ++j) {
         const uint32_t slot = (uint32_t)j * kBlock + threadIdx.x;
-        if (slot < nrec)
-          stage_put<L, (1 << PB)>(st, slot, cur[j], (uint32_t)((mix64(rec_key(cur[j], kb)) >> shift) & dmask));
+        dg[j] = (uint32_t)((mix64(rec_key(cur[j], kb)) >> shift) & dmask);
+        rk[j] = 0;
+        if (slot < nrec) rk[j] = atomicAdd(&st.count[dg[j]], 1u);
       }
-      stage_flush<L, (1 << PB)>(st, nrec, D, out, cap, counters);
+      lds_barrier();
+      scan_digits(st.count, st.start, D, st.wave_tot);
+      // 2. place the records sorted by digit
+#pragma unroll
+      for (int j = 0; j < PT; ++j) {
+        const uint32_t slot = (uint32_t)j * kBlock + threadIdx.x;
+        if (slot < nrec) {
+          const uint32_t at = st.start[dg[j]] + rk[j];
+          st.rec[at] = cur[j];
+          st.digit[at] = (std::remove_reference_t<decltype(st.digit[0])>)dg[j];
+        }
+      }
+      lds_barrier();
+      // 3. linear write-out of the digit runs
+      for (uint32_t i = threadIdx.x; i < nrec; i += kBlock) {
+        const uint32_t d = st.digit[i];
+        const uint64_t pos = st.gpos[d] + (i - st.start[d]);
+        if (pos < cap)
+          out[pos] = st.rec[i];
+        else
+          atomicOr((unsigned long long*)&counters[kError], 2ull);
+      }
+      lds_barrier();
+      for (uint32_t d = threadIdx.x; d < D; d += kBlock) {
+        st.gpos[d] += st.count[d];
+        st.count[d] = 0;
+      }
+      lds_barrier();
     }
   }
 }
