@@ -220,9 +220,11 @@ __device__ __forceinline__ void scan_digits(const uint32_t* counts, uint32_t* st
   lds_barrier();
 }
 
-// LDS staging area of the expand-scatter and partition kernels: records are
-// appended with a per-bucket rank, then written out grouped by bucket so
-// consecutive threads store consecutive addresses of a bucket run.
+// LDS staging area of the seed-row expand-scatter: records are appended
+// with a per-bucket rank, then written out grouped by bucket so consecutive
+// threads store consecutive addresses of a bucket run.  (A thread emits a
+// variable number of successors there, so ranks cannot stay in registers as
+// in the partition's PartStage.)
 template <int L, int DMAX>
 struct Stage {
   Rec<L> rec[kStage];

@@ -113,3 +113,22 @@ def test_batched_and_graph_replay_in_both_formats():
 def test_prefix_counts_packed_match_dfs():
     for cols in ([0], [1, 3], [2, 0, 3]):
         assert pm.count_prefix(12, cols, opts(PEPS_RECORDS_PACKED)) == o.count_prefix_dfs(12, cols), cols
+
+
+def test_published_crt_release_n18_n19():
+    # benchmarks/e45_wide_crt_release.csv rows for N = 18 and 19: counts and
+    # both CRT residues.  N = 18 through the default (packed) plain count;
+    # N = 19's plain frontier (1.25e10 candidates) does not fit one GPU's
+    # two candidate buffers, so it runs as 2-row prefix slices.
+    import json
+    import os
+    fix = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "reference_fixtures.json")))
+    rows = {r["n"]: r for r in fix["crt_release"]}
+    for n in (18, 19):
+        show("crt release", n)
+        g = pm.contract_rows(n) if n == 18 else pm.count_batched(n, 2)[0]
+        row = rows[n]
+        assert g.record_bytes == 12
+        assert g.count.value == row["count"], row["src"]
+        assert list(g.primes)[:g.crt_lanes] == row["primes"], row["src"]
+        assert list(g.residues)[:g.crt_lanes] == row["residues"], row["src"]
