@@ -32,7 +32,7 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, vendor figure
 # rocprofv3 kernel names of the engine's profiling labels (L: CRT lanes of
 # the record, 0 for packed 12-byte records; T: LDS table slots).
 ROCPROF_NAMES = {
-    "merge_count": "k_merge_expand<{L}, 0, {T}, 10, 256>",
+    "merge_count": "k_merge_expand<{L}, 0, {T}, 8, 256>",
     "merge_scatter": "k_merge_expand<{L}, 1, {T}, 8, 512>",
     "merge_last": "k_merge_expand<{L}, 2, {T}, 10, 256>",
     "part_hist": "k_part_hist<{L}, 8>",

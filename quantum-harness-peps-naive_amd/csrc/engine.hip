@@ -829,8 +829,10 @@ int Engine::pipeline(const RunSpec& spec, Plan* plan, bool eager, RunOut* out, s
                                    nullptr, nullptr, nullptr, 0, final_mask, ssum, nslice / L, cr, table_slots,
                                    stream_));
       } else {
-        // COUNT: successor digits of the merged states at the fine resolution.
-        const uint32_t DF = 1u << kFineBits;
+        // COUNT: successor digits of the merged states at the widest digit
+        // the SCATTER may write (coarsened below once the total is known).
+        const uint32_t CB = std::min<uint32_t>(kFineBits, std::max(1u, spec.expand_bits));
+        const uint32_t DF = 1u << CB;
         const int nsidx = sidx ^ 1;
         rc = need(&hist_, &hist_cap_, &plan->hist_need, (size_t)DF * W);
         if (!rc) rc = need(&offsets_, &offsets_cap_, &plan->offsets_need, (size_t)DF * W);
@@ -840,7 +842,7 @@ int Engine::pipeline(const RunSpec& spec, Plan* plan, bool eager, RunOut* out, s
         uint64_t* cn = row_counters(row + 1);
         LAUNCH("merge_count", 8.0 * C,
                launch_merge_expand(kMergeCount, LK, region_[cur], segs_[sidx], (uint32_t)nseg, W, g, pr,
-                                   kFineBits, hist_, nullptr, nullptr, 0, 0, nullptr, 0, cr, table_slots, stream_));
+                                   CB, hist_, nullptr, nullptr, 0, 0, nullptr, 0, cr, table_slots, stream_));
         LAUNCH("scan", 12.0 * DF * W, launch_scan(hist_, DF, W, 1, offsets_, segs_[nsidx], partial_, cn, stream_));
         if (eager) {
           HIP_TRY(hipMemcpyAsync(h_counters_, cn, sizeof(uint64_t) * kNumCounters, hipMemcpyDeviceToHost,
@@ -856,14 +858,14 @@ int Engine::pipeline(const RunSpec& spec, Plan* plan, bool eager, RunOut* out, s
           std::vector<unsigned> pd = rp.pdigits;
           if (eager) {
             const unsigned B = total_bits(C_next);
-            d1 = std::min<unsigned>(B, std::min<unsigned>(kFineBits, std::max(1u, spec.expand_bits)));
+            d1 = std::min<unsigned>(B, CB);
             pd = split_bits(B, d1);
             rp.d1 = d1;
             rp.pdigits = pd;
           }
-          if (d1 < kFineBits)
+          if (d1 < CB)
             LAUNCH("scan", 12.0 * DF * W,
-                   launch_scan(hist_, 1u << d1, W, 1u << (kFineBits - d1), offsets_, segs_[nsidx], partial_, cn,
+                   launch_scan(hist_, 1u << d1, W, 1u << (CB - d1), offsets_, segs_[nsidx], partial_, cn,
                                stream_));
           const int to = cur ^ 1;
           rc = grow(to, C_next * rb);

@@ -1332,7 +1332,12 @@ static hipError_t launch_me(int mode, dim3 grid, hipStream_t stream, const void*
   const Rec<L>* i = (const Rec<L>*)in;
   Rec<L>* o = (Rec<L>*)out;
   switch (mode) {
-    case kMergeCount: hipLaunchKernelGGL((k_merge_expand<L, kMergeCount, T>), grid, dim3(kBlock), 0, stream, i, segs, nseg, per, g, pr, digit_bits, hist, offsets, o, cap, final_mask, slice_sums, nslices, counters); break;
+    case kMergeCount:
+      if (digit_bits <= kNarrowDigitBits)
+        hipLaunchKernelGGL((k_merge_expand<L, kMergeCount, T, kNarrowDigitBits>), grid, dim3(kBlock), 0, stream, i, segs, nseg, per, g, pr, digit_bits, hist, offsets, o, cap, final_mask, slice_sums, nslices, counters);
+      else
+        hipLaunchKernelGGL((k_merge_expand<L, kMergeCount, T>), grid, dim3(kBlock), 0, stream, i, segs, nseg, per, g, pr, digit_bits, hist, offsets, o, cap, final_mask, slice_sums, nslices, counters);
+      break;
     case kMergeScatter:
       if (digit_bits <= kNarrowDigitBits)
         hipLaunchKernelGGL((k_merge_expand<L, kMergeScatter, T, kNarrowDigitBits, kScatterBlock>), grid, dim3(kScatterBlock), 0, stream, i, segs, nseg, per, g, pr, digit_bits, hist, offsets, o, cap, final_mask, slice_sums, nslices, counters);
