@@ -2,8 +2,10 @@
 //
 // Requests are (N, prefix) counts.  Each is split into prefix tasks at a
 // common start row D = max(k, min(2, N-1)) so that requests of one N (and one
-// D) can share a launch: their tasks become the slices of one row pipeline
-// and every slice's final sum comes back separately.  One worker thread per
+// D) can share a launch.  Within a launch every request is one slice: the
+// tasks of a request share its slice id, so their frontiers merge exactly as
+// in that request's own count, while different requests never merge, and
+// every request's final sum comes back separately.  One worker thread per
 // server picks a class, fills a launch from every active request of that
 // class, runs it on the engine (whose frontiers stay in HBM), credits the
 // slice counts to their requests and repeats; new requests join at the next
@@ -18,6 +20,7 @@
 #include <mutex>
 #include <string>
 #include <thread>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/peps_mi355x.h"
@@ -42,6 +45,37 @@ struct Request {
   double t_submit = 0, t_start = -1, t_finish = -1;
   uint32_t launches = 0, max_batch = 0;
 };
+
+// Tasks [a, b) of a launch as one row pipeline with one slice per owning
+// request (slice ids in order of first appearance); seeds of one request
+// that reach the same boundary are merged.  *slice_owner[s] is slice s's
+// request.
+peps::RunSpec request_slices_spec(unsigned n, unsigned depth, const std::vector<peps::Task>& tasks,
+                                  const std::vector<Request*>& owner, size_t a, size_t b,
+                                  std::vector<Request*>* slice_owner) {
+  peps::RunSpec spec;
+  spec.n = n;
+  spec.first_row = depth;
+  spec.orbit_first_row = depth == 0;
+  spec.per_slice = true;
+  slice_owner->clear();
+  std::unordered_map<Request*, uint64_t> sid;
+  std::unordered_map<uint64_t, size_t> at;  // key | slice << 3n -> seed index
+  for (size_t i = a; i < b; ++i) {
+    auto ins = sid.emplace(owner[i], slice_owner->size());
+    if (ins.second) slice_owner->push_back(owner[i]);
+    const uint64_t key = tasks[i].key | (ins.first->second << (3 * n));
+    auto seed = at.emplace(key, spec.seeds.size());
+    if (seed.second)
+      spec.seeds.push_back(peps::Seed{key, tasks[i].weight});
+    else
+      spec.seeds[seed.first->second].weight += tasks[i].weight;
+  }
+  unsigned bits = 0;
+  while (((size_t)1 << bits) < slice_owner->size()) ++bits;
+  spec.slice_bits = bits;
+  return spec;
+}
 
 }  // namespace
 
@@ -151,17 +185,40 @@ struct peps_server {
       }
       lk.unlock();
 
-      // Slice counts of each sub-launch, credited after the lock is retaken.
-      std::vector<u128> counts(batch.size(), 0);
+      // Per-request counts of each sub-launch, credited after the lock is
+      // retaken.  A launch that does not fit the HBM budget is split in two
+      // and retried.
+      std::vector<std::pair<Request*, u128>> credit;
       std::vector<std::pair<size_t, size_t>> ranges;
       std::vector<peps::RunOut> outs;
-      auto done = [&](size_t a, size_t b, const peps::RunOut& ro) {
-        for (size_t i = a; i < b; ++i) counts[i] = ro.slice_counts[i - a];
-        ranges.emplace_back(a, b);
-        outs.push_back(ro);
-        outs.back().slice_counts.clear();
-      };
-      const int rc = peps::run_task_range(res, n, depth, batch, 0, batch.size(), done, &e);
+      int rc = PEPS_OK;
+      {
+        std::vector<std::pair<size_t, size_t>> work{{0, batch.size()}};
+        while (!work.empty() && rc == PEPS_OK) {
+          const auto [a, b] = work.back();
+          work.pop_back();
+          std::vector<Request*> slice_owner;
+          peps::RunSpec spec = request_slices_spec(n, depth, batch, owner, a, b, &slice_owner);
+          peps::RunOut ro;
+          std::string se;
+          rc = peps::run_engine(res, spec, &ro, &se);
+          if (rc == PEPS_E_MEMORY && b - a > 1) {
+            const size_t mid = a + (b - a) / 2;
+            work.emplace_back(mid, b);
+            work.emplace_back(a, mid);
+            rc = PEPS_OK;
+            continue;
+          }
+          if (rc) {
+            e = se;
+            break;
+          }
+          for (size_t sl = 0; sl < slice_owner.size(); ++sl) credit.emplace_back(slice_owner[sl], ro.slice_counts[sl]);
+          ranges.emplace_back(a, b);
+          ro.slice_counts.clear();
+          outs.push_back(std::move(ro));
+        }
+      }
       const double t_end = now();
 
       lk.lock();
@@ -185,14 +242,16 @@ struct peps_server {
         }
         continue;
       }
+      for (const auto& [r, c] : credit) {
+        if (r->status != PEPS_E_PENDING) continue;
+        if (!peps::checked_add(r->count, c, &r->count)) {
+          runnable.erase(std::remove(runnable.begin(), runnable.end(), r->id), runnable.end());
+          finish(*r, PEPS_E_OVERFLOW, "count overflow");
+        }
+      }
       for (size_t i = 0; i < batch.size(); ++i) {
         Request* r = owner[i];
         if (r->status != PEPS_E_PENDING) continue;
-        if (!peps::checked_add(r->count, counts[i], &r->count)) {
-          runnable.erase(std::remove(runnable.begin(), runnable.end(), r->id), runnable.end());
-          finish(*r, PEPS_E_OVERFLOW, "count overflow");
-          continue;
-        }
         if (++r->finished == r->tasks.size()) {
           u128 fact;
           peps::factorial(r->n, &fact);
