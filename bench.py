@@ -35,8 +35,11 @@ ROCPROF_NAMES = {
     "merge_count": "k_merge_expand<{L}, 0, {T}, 8, 256>",
     "merge_scatter": "k_merge_expand<{L}, 1, {T}, 8, 512>",
     "merge_last": "k_merge_expand<{L}, 2, {T}, 10, 256>",
-    "part_hist": "k_part_hist<{L}, 8>",
-    "part_scatter": "k_part_scatter<{L}, 8>",
+    "count_ub": "k_count_ub<{L}>",
+    "part_hist": "k_part_hist<{L}, 8, false>",
+    "part_hist_runs": "k_part_hist<{L}, 8, true>",
+    "part_scatter": "k_part_scatter<{L}, 8, false>",
+    "part_scatter_runs": "k_part_scatter<{L}, 8, true>",
 }
 
 

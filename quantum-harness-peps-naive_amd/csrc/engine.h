@@ -101,6 +101,8 @@ struct RunOut {
 // captured once into a HIP graph and replayed without host synchronisation.
 struct RowPlan {
   uint64_t C_next = 0;             // candidates of the next row (0: none)
+  uint64_t C_count = 0;            // what the COUNT pass reported (C_next, or its upper bound)
+  bool upper_bound = false;        // COUNT without merging; SCATTER leaves gaps, level 1 reads runs
   uint32_t W = 0;                  // merge-expand workgroups for this row
   uint32_t d1 = 0;                 // next row's level-1 digit width
   std::vector<unsigned> pdigits;   // next row's partition digit widths
@@ -115,6 +117,7 @@ struct Plan {
   std::vector<RowPlan> rows;       // rows [first_row, n)
   size_t region_need = 0, hist_need = 0, offsets_need = 0, partial_need = 0;
   size_t segs_need[2] = {0, 0};
+  size_t run_start_need = 0, run_len_need = 0;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   uint64_t generation = 0;         // buffer generation the graph was captured against
@@ -198,6 +201,10 @@ class Engine {
   size_t segs_cap_[2] = {0, 0};
   uint64_t* partial_ = nullptr;
   size_t partial_cap_ = 0;
+  uint64_t* run_start_ = nullptr;      // upper-bound rows: SCATTER run starts [D1][W]
+  size_t run_start_cap_ = 0;
+  uint32_t* run_len_ = nullptr;        // upper-bound rows: SCATTER run lengths [D1][W]
+  size_t run_len_cap_ = 0;
   uint64_t* counters_rows_ = nullptr;  // per-row counters, (n + 1) * kNumCounters
   size_t counters_rows_cap_ = 0;
   uint64_t* seed_ = nullptr;           // seed frontier (SoA, seed_image layout)

@@ -241,6 +241,9 @@ namespace {
 // Internal return code: a packed record's weight outgrew its field.
 constexpr int kRcPackedOverflow = 100;
 
+// Rows with at least this many candidates use the upper-bound COUNT.
+constexpr uint64_t kUpperBoundRows = (uint64_t)1 << 20;
+
 bool packed_records(const RunSpec& s) { return s.records != kRecordsLanes && !s.materialize && !s.site; }
 
 // Key and weight widths of the packed records of a run.
@@ -382,6 +385,8 @@ int Engine::run_many_once(const std::string& key, const std::vector<RunSpec>& sp
     if (!rc) rc = ensure_buffer(&partial_, &partial_cap_, plan.partial_need, err);
     if (!rc) rc = ensure_buffer(&segs_[0], &segs_cap_[0], plan.segs_need[0], err);
     if (!rc) rc = ensure_buffer(&segs_[1], &segs_cap_[1], plan.segs_need[1], err);
+    if (!rc) rc = ensure_buffer(&run_start_, &run_start_cap_, plan.run_start_need, err);
+    if (!rc) rc = ensure_buffer(&run_len_, &run_len_cap_, plan.run_len_need, err);
   }
   if (rc) return rc;
   // Per-group seeds, counters and slice sums, so that every group's pipeline
@@ -540,6 +545,8 @@ int Engine::run_once(const RunSpec& spec, RunOut* out, std::string* err) {
       if (!rc) rc = ensure_buffer(&partial_, &partial_cap_, plan.partial_need, err);
       if (!rc) rc = ensure_buffer(&segs_[0], &segs_cap_[0], plan.segs_need[0], err);
       if (!rc) rc = ensure_buffer(&segs_[1], &segs_cap_[1], plan.segs_need[1], err);
+      if (!rc) rc = ensure_buffer(&run_start_, &run_start_cap_, plan.run_start_need, err);
+      if (!rc) rc = ensure_buffer(&run_len_, &run_len_cap_, plan.run_len_need, err);
       if (!rc) rc = ensure_buffer(&counters_rows_, &counters_rows_cap_, (size_t)(n + 1) * kNumCounters, err);
       if (!rc) rc = ensure_buffer(&seed_, &seed_cap_, seed_host.size(), err);
       if (!rc && spec.per_slice)
@@ -736,15 +743,21 @@ int Engine::pipeline(const RunSpec& spec, Plan* plan, bool eager, RunOut* out, s
   uint64_t S_in = S0;
   unsigned levels = 0;
 
-  auto partition_to_final = [&](int from, unsigned d1, const std::vector<unsigned>& pd, uint64_t Crow) -> int {
+  // Partition levels after the level-1 digit d1.  With `runs`, the first
+  // level reads the SCATTER's runs (upper-bound offsets, gaps between runs)
+  // and its scan writes the exact candidate count into `exact`.
+  auto partition_to_final = [&](int from, unsigned d1, const std::vector<unsigned>& pd, uint64_t Crow,
+                                const Runs* runs, uint64_t* exact) -> int {
     cur = from;
     nseg = (uint64_t)1 << d1;
     levels = d1 ? 1 : 0;
     unsigned used = d1;
+    const Runs* rn = runs;  // the first level only
     for (unsigned dk : pd) {
       used += dk;
       const int to = cur ^ 1;
-      const uint32_t chunks = partition_chunks(Crow, nseg);
+      uint32_t chunks = partition_chunks(Crow, nseg);
+      if (rn) chunks = std::max<uint32_t>(chunks, (rn->per_seg + kMaxChunkRuns - 1) / kMaxChunkRuns);
       const size_t cells = (size_t)(nseg << dk) * chunks;
       int e = grow(to, Crow * rb);
       if (!e) e = need(&segs_[sidx ^ 1], &segs_cap_[sidx ^ 1], &plan->segs_need[sidx ^ 1], (nseg << dk) + 1);
@@ -753,19 +766,20 @@ int Engine::pipeline(const RunSpec& spec, Plan* plan, bool eager, RunOut* out, s
       if (!e) e = need(&partial_, &partial_cap_, &plan->partial_need, cells / 4096 + 2);
       if (e) return e;
       uint64_t* spare = row_counters(0) + n * kNumCounters;
-      LAUNCH("part_hist", 8.0 * Crow,
+      LAUNCH(rn ? "part_hist_runs" : "part_hist", 8.0 * Crow,
              launch_part_hist(LK, region_[cur], segs_[sidx], (uint32_t)nseg, chunks, 64 - used, dk, g.key_bits,
-                              hist_, stream_));
+                              rn, hist_, stream_));
       LAUNCH("scan", 12.0 * cells,
-             launch_scan(hist_, (uint32_t)(nseg << dk), chunks, 1, offsets_, segs_[sidx ^ 1], partial_, spare,
-                         stream_));
-      LAUNCH("part_scatter", 2.0 * R * Crow,
+             launch_scan(hist_, (uint32_t)(nseg << dk), chunks, 1, offsets_, segs_[sidx ^ 1], partial_,
+                         rn ? exact : spare, stream_));
+      LAUNCH(rn ? "part_scatter_runs" : "part_scatter", 2.0 * R * Crow,
              launch_part_scatter(LK, region_[cur], region_[to], segs_[sidx], (uint32_t)nseg, chunks, 64 - used, dk,
-                                 g.key_bits, offsets_, region_bytes_[to] / rb, spare, stream_));
+                                 g.key_bits, rn, offsets_, region_bytes_[to] / rb, spare, stream_));
       cur = to;
       sidx ^= 1;
       nseg <<= dk;
       levels += 1;
+      rn = nullptr;
     }
     return 0;
   };
@@ -800,7 +814,7 @@ int Engine::pipeline(const RunSpec& spec, Plan* plan, bool eager, RunOut* out, s
            launch_expand_scatter(LK, skeys, sres, S0, S0, g, pr, d1, (uint32_t)tiles, per_tile, offsets_,
                                  region_[0], region_bytes_[0] / rb, row_counters(0) + n * kNumCounters,
                                  stream_));
-    rc = partition_to_final(0, d1, pd, C);
+    rc = partition_to_final(0, d1, pd, C, nullptr, nullptr);
     if (rc) return rc;
   }
   g.orbit_row = 0;
@@ -831,53 +845,94 @@ int Engine::pipeline(const RunSpec& spec, Plan* plan, bool eager, RunOut* out, s
       } else {
         // COUNT: successor digits of the merged states at the widest digit
         // the SCATTER may write (coarsened below once the total is known).
+        // Large rows count the successors of every record instead (an upper
+        // bound per digit and workgroup, no LDS merge); the SCATTER then
+        // leaves a gap after each of its runs and the first partition level
+        // reads the runs.
         const uint32_t CB = std::min<uint32_t>(kFineBits, std::max(1u, spec.expand_bits));
         const uint32_t DF = 1u << CB;
+        const bool ub = eager ? C >= kUpperBoundRows : rp.upper_bound;
+        rp.upper_bound = ub;
         const int nsidx = sidx ^ 1;
         rc = need(&hist_, &hist_cap_, &plan->hist_need, (size_t)DF * W);
         if (!rc) rc = need(&offsets_, &offsets_cap_, &plan->offsets_need, (size_t)DF * W);
         if (!rc) rc = need(&partial_, &partial_cap_, &plan->partial_need, ((size_t)DF * W) / 4096 + 2);
         if (!rc) rc = need(&segs_[nsidx], &segs_cap_[nsidx], &plan->segs_need[nsidx], (size_t)DF + 1);
+        if (!rc && ub) rc = need(&run_start_, &run_start_cap_, &plan->run_start_need, (size_t)DF * W);
+        if (!rc && ub) rc = need(&run_len_, &run_len_cap_, &plan->run_len_need, (size_t)DF * W);
         if (rc) return rc;
+        // SCATTER write offsets, taken only after every buffer above is sized
+        // (a growing buffer moves).
+        uint64_t* const wpos = ub ? run_start_ : offsets_;
         uint64_t* cn = row_counters(row + 1);
-        LAUNCH("merge_count", 8.0 * C,
-               launch_merge_expand(kMergeCount, LK, region_[cur], segs_[sidx], (uint32_t)nseg, W, g, pr,
-                                   CB, hist_, nullptr, nullptr, 0, 0, nullptr, 0, cr, table_slots, stream_));
-        LAUNCH("scan", 12.0 * DF * W, launch_scan(hist_, DF, W, 1, offsets_, segs_[nsidx], partial_, cn, stream_));
+        if (ub)
+          LAUNCH("count_ub", 8.0 * C,
+                 launch_count_ub(LK, region_[cur], segs_[sidx], (uint32_t)nseg, W, g, CB, hist_, stream_));
+        else
+          LAUNCH("merge_count", 8.0 * C,
+                 launch_merge_expand(kMergeCount, LK, region_[cur], segs_[sidx], (uint32_t)nseg, W, g, pr,
+                                     CB, hist_, nullptr, nullptr, 0, 0, nullptr, 0, cr, table_slots, stream_));
+        LAUNCH("scan", 12.0 * DF * W, launch_scan(hist_, DF, W, 1, wpos, segs_[nsidx], partial_, cn, stream_));
+        uint64_t C_count = 0;  // candidates the SCATTER may write (exact, or the upper bound)
         if (eager) {
           HIP_TRY(hipMemcpyAsync(h_counters_, cn, sizeof(uint64_t) * kNumCounters, hipMemcpyDeviceToHost,
                                  stream_));
           SYNC_AND_RESOLVE();
-          C_next = h_counters_[kCandidates];
-          rp.C_next = C_next;
+          C_count = h_counters_[kCandidates];
+          rp.C_count = C_count;
         } else {
-          C_next = rp.C_next;
+          C_count = rp.C_count;
         }
-        if (C_next) {
+        if (C_count) {
           unsigned d1 = rp.d1;
           std::vector<unsigned> pd = rp.pdigits;
           if (eager) {
-            const unsigned B = total_bits(C_next);
+            const unsigned B = total_bits(C_count);
             d1 = std::min<unsigned>(B, CB);
+            // Upper-bound rows need at least one partition level to close
+            // the gaps between the SCATTER's runs (their levels are planned
+            // from the exact count once the SCATTER has run).
+            if (ub) d1 = B ? std::min<unsigned>(B - 1, CB) : 0;
             pd = split_bits(B, d1);
             rp.d1 = d1;
             rp.pdigits = pd;
           }
           if (d1 < CB)
             LAUNCH("scan", 12.0 * DF * W,
-                   launch_scan(hist_, 1u << d1, W, 1u << (CB - d1), offsets_, segs_[nsidx], partial_, cn,
-                               stream_));
+                   launch_scan(hist_, 1u << d1, W, 1u << (CB - d1), wpos, segs_[nsidx], partial_, cn, stream_));
           const int to = cur ^ 1;
-          rc = grow(to, C_next * rb);
+          rc = grow(to, C_count * rb);
           if (rc) return rc;
-          LAUNCH("merge_scatter", R * (C + C_next),
+          LAUNCH("merge_scatter", R * (C + C_count),
                  launch_merge_expand(kMergeScatter, LK, region_[cur], segs_[sidx], (uint32_t)nseg, W, g, pr, d1,
-                                     nullptr, offsets_, region_[to], region_bytes_[to] / rb, final_mask, ssum,
-                                     nslice / L, cr, table_slots, stream_));
+                                     ub ? run_len_ : nullptr, wpos, region_[to], region_bytes_[to] / rb, final_mask,
+                                     ssum, nslice / L, cr, table_slots, stream_));
           sidx = nsidx;
-          rc = partition_to_final(to, d1, pd, C_next);
+          // The exact count: from COUNT, or (upper-bound rows) what the
+          // SCATTER wrote, which also corrects its profiled bytes.
+          C_next = C_count;
+          if (ub) {
+            if (eager) {
+              HIP_TRY(hipMemcpyAsync(h_counters_, cr, sizeof(uint64_t) * kNumCounters, hipMemcpyDeviceToHost,
+                                     stream_));
+              HIP_TRY(hipStreamSynchronize(stream_));
+              C_next = h_counters_[kWritten];
+              if (profiling_ && !pending_.empty() && pending_.back().kernel == kernel_id("merge_scatter"))
+                pending_.back().bytes = R * (double)(C + C_next);
+              SYNC_AND_RESOLVE();
+              pd = split_bits(std::max(total_bits(C_next), d1 + 1), d1);
+              rp.pdigits = pd;
+            } else {
+              C_next = rp.C_next;
+            }
+          }
+          rp.C_next = C_next;
+          const Runs runs{run_start_, run_len_, W};
+          rc = partition_to_final(to, d1, pd, C_next, ub ? &runs : nullptr, cn);
           if (rc) return rc;
         } else {
+          C_next = 0;
+          rp.C_next = 0;
           // No successor survives: merge for the statistics only.
           LAUNCH("merge_last", R * C,
                  launch_merge_expand(kMergeLast, LK, region_[cur], segs_[sidx], (uint32_t)nseg, W, g, pr, 0,
@@ -1245,14 +1300,14 @@ int Engine::materialized(const RunSpec& spec, RunOut* out, std::string* err) {
         if (!rc) rc = ensure_buffer(&partial_, &partial_cap_, cells / 4096 + 2, err);
         if (rc) return rc;
         LAUNCH("part_hist", 8.0 * C,
-               launch_part_hist(L, region_[cur], segs_[sidx], (uint32_t)nseg, chunks, 64 - used, dk, 0, hist_,
-                                stream_));
+               launch_part_hist(L, region_[cur], segs_[sidx], (uint32_t)nseg, chunks, 64 - used, dk, 0, nullptr,
+                                hist_, stream_));
         LAUNCH("scan", 12.0 * cells,
                launch_scan(hist_, (uint32_t)(nseg << dk), chunks, 1, offsets_, segs_[sidx ^ 1], partial_, spare,
                            stream_));
         LAUNCH("part_scatter", 2.0 * R * C,
                launch_part_scatter(L, region_[cur], region_[to], segs_[sidx], (uint32_t)nseg, chunks, 64 - used,
-                                   dk, 0, offsets_, region_bytes_[to] / rb, spare, stream_));
+                                   dk, 0, nullptr, offsets_, region_bytes_[to] / rb, spare, stream_));
         cur = to;
         sidx ^= 1;
         nseg <<= dk;

@@ -69,7 +69,8 @@ enum Counter : int {
   kSiteVisits = 12,  // sitewise: partial-row visits (one per partial per site)
   kSiteMatched = 13, // sitewise: C entries applied (= entries examined through the input index)
   kEmitted = 14,     // EMIT merge: merged states written to the frontier
-  kNumCounters = 15
+  kWritten = 15,     // SCATTER merge: successors written (the next row's exact candidates)
+  kNumCounters = 16
 };
 
 // COUNT / SCATTER expand the merged states into the next row; LAST only sums;
@@ -144,19 +145,39 @@ hipError_t launch_scan(const uint32_t* hist, uint32_t D, uint32_t W, uint32_t gr
 constexpr uint64_t kPartChunk = 32768;     // target records per partition workgroup
 constexpr uint64_t kMaxPartChunks = 4096;
 uint32_t partition_chunks(uint64_t records, uint64_t nseg);
+// Run mode of a partition level (its input is the SCATTER output with
+// upper-bound offsets): segment s is the runs w < per_seg, run (s, w) being
+// the records [start[s * per_seg + w], start[...] + len[s * per_seg + w]).
+// Chunk p takes the runs [per_seg * p / chunks, per_seg * (p + 1) / chunks),
+// at most kMaxChunkRuns of them, so chunks >= ceil(per_seg / kMaxChunkRuns).
+constexpr uint32_t kMaxChunkRuns = 64;
+struct Runs {
+  const uint64_t* start;
+  const uint32_t* len;
+  uint32_t per_seg;
+};
 // The partition is three launches: hist, launch_scan(hist, nseg << digit_bits,
 // chunks, 1, offsets, subsegs, partial, counters), scatter.
-// key_bits: the packed records' key width (ignored for lane records).
+// key_bits: the packed records' key width (ignored for lane records).  runs:
+// run mode (segs unused), or null for contiguous segments.
 hipError_t launch_part_hist(int lanes, const void* in, const uint64_t* segs, uint32_t nseg, uint32_t chunks,
-                            uint32_t shift, uint32_t digit_bits, uint32_t key_bits, uint32_t* hist,
-                            hipStream_t stream);
+                            uint32_t shift, uint32_t digit_bits, uint32_t key_bits, const Runs* runs,
+                            uint32_t* hist, hipStream_t stream);
 hipError_t launch_part_scatter(int lanes, const void* in, void* out, const uint64_t* segs, uint32_t nseg,
                                uint32_t chunks, uint32_t shift, uint32_t digit_bits, uint32_t key_bits,
-                               const uint64_t* offsets, uint64_t cap, uint64_t* counters, hipStream_t stream);
+                               const Runs* runs, const uint64_t* offsets, uint64_t cap, uint64_t* counters,
+                               hipStream_t stream);
+
+// Upper-bound COUNT: successor digits of every record of the buckets (no
+// merge), per merge workgroup of a W-workgroup merge launch, hist[d * W + w].
+hipError_t launch_count_ub(int lanes, const void* in, const uint64_t* segs, uint32_t nseg, uint32_t W,
+                           const RowGeom& g, uint32_t digit_bits, uint32_t* hist, hipStream_t stream);
 
 // Fused merge of row r's buckets with the expansion into row r+1.
 // COUNT: hist[(1<<kFineBits)][W] of successor digits.  SCATTER: successors
-// written to out (capacity `cap` records) at offsets[(1<<digit_bits)][W].
+// written to out (capacity `cap` records) at offsets[(1<<digit_bits)][W]; if
+// hist is given, the number written per (digit, workgroup) goes there (the
+// run lengths when the offsets are upper bounds).
 // LAST: no expansion.  EMIT (lane records only): the merged states are
 // written to `out` as a frontier (keys[cap] then residues[l * cap + i]),
 // positions counted in counters[kEmitted].  Merged states with (key & final_mask) == final_mask

@@ -320,6 +320,40 @@ __global__ __launch_bounds__(kBlock) void k_expand_hist(const uint64_t* __restri
   for (uint32_t i = threadIdx.x; i < D; i += kBlock) hist[(uint64_t)i * gridDim.x + blockIdx.x] = lh[i];
 }
 
+// ------------------------------------------------------ upper-bound count --
+// Successor digits of every record of row r's buckets, duplicates included,
+// per merge workgroup (the same bucket assignment as the merges): an upper
+// bound of what the SCATTER merge writes per (digit, workgroup), without
+// the LDS table.  hist[d * W + w].
+template <int L>
+__global__ __launch_bounds__(kBlock) void k_count_ub(const Rec<L>* __restrict__ in,
+                                                     const uint64_t* __restrict__ segs, uint32_t nseg,
+                                                     uint32_t per, RowGeom g, uint32_t digit_bits,
+                                                     uint32_t* __restrict__ hist) {
+  __shared__ uint32_t lh[kExpandDigits];
+  const uint32_t D = 1u << digit_bits;
+  const uint32_t W = gridDim.x;
+  for (uint32_t i = threadIdx.x; i < D; i += kBlock) lh[i] = 0;
+  lds_barrier();
+  const uint32_t b0 = blockIdx.x * per;
+  const uint32_t b1 = (b0 + per < nseg) ? b0 + per : nseg;
+  if (b0 < b1) {
+    const uint64_t lo = segs[b0], hi = segs[b1];
+    for (uint64_t i = lo + threadIdx.x; i < hi; i += kBlock) {
+      const uint64_t key = rec_key(in[i], g.key_bits);
+      const Board bs = unpack(key, g);
+      uint32_t free = legal_positions(bs, g);
+      while (free) {
+        const uint32_t b = free & (~free + 1);
+        free ^= b;
+        atomicAdd(&lh[top_digit(successor(key, bs, b, g), digit_bits)], 1u);
+      }
+    }
+  }
+  lds_barrier();
+  for (uint32_t i = threadIdx.x; i < D; i += kBlock) hist[(uint64_t)i * W + blockIdx.x] = lh[i];
+}
+
 // --------------------------------------------------------- expand scatter --
 
 template <int L>
@@ -673,27 +707,86 @@ __device__ __forceinline__ void chunk_range(const uint64_t* __restrict__ segs, u
   *b = lo + len * (p + 1) / chunks;
 }
 
-template <int L, int PB>
+// Run mode: chunk p of segment s is the runs [per_seg * p / chunks,
+// per_seg * (p + 1) / chunks) of that segment (at most kMaxChunkRuns), each a
+// contiguous range of records.  The table holds their starts and the
+// exclusive prefix sums of their lengths; pre[kMaxChunkRuns] is the total.
+struct RunTable {
+  uint64_t start[kMaxChunkRuns];
+  uint32_t pre[kMaxChunkRuns + 1];
+};
+
+// Fills `t` with wave 0 (the caller then waits at a barrier); returns the
+// number of runs of the chunk.
+__device__ __forceinline__ uint32_t load_runs(const Runs& rn, uint32_t s, uint32_t p, uint32_t chunks,
+                                              RunTable& t) {
+  const uint32_t r0 = (uint32_t)((uint64_t)rn.per_seg * p / chunks);
+  const uint32_t r1 = (uint32_t)((uint64_t)rn.per_seg * (p + 1) / chunks);
+  const uint32_t nr = r1 - r0;
+  if (threadIdx.x < 64) {
+    const uint32_t lane = threadIdx.x;
+    uint32_t len = 0;
+    if (lane < nr) {
+      const uint64_t idx = (uint64_t)s * rn.per_seg + r0 + lane;
+      len = rn.len[idx];
+      t.start[lane] = rn.start[idx];
+    }
+    uint32_t inc = len;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t y = __shfl_up(inc, o, 64);
+      if (lane >= (uint32_t)o) inc += y;
+    }
+    if (lane < nr) t.pre[lane] = inc - len;
+    if (lane == 63) t.pre[kMaxChunkRuns] = inc;
+  }
+  return nr;
+}
+
+// Record index of the chunk's j-th record (j < total).  `r` is the thread's
+// run cursor (0 at the chunk's start; a thread's j only grow): it moves to
+// the last run whose prefix is <= j, which holds j (empty runs share their
+// successor's prefix and are stepped over).
+__device__ __forceinline__ uint64_t run_record(const RunTable& t, uint32_t nr, uint32_t j, uint32_t& r) {
+  while (r + 1 < nr && t.pre[r + 1] <= j) ++r;
+  return t.start[r] + (j - t.pre[r]);
+}
+
+template <int L, int PB, bool RUNS>
 __global__ __launch_bounds__(kBlock) void k_part_hist(const Rec<L>* __restrict__ in,
                                                       const uint64_t* __restrict__ segs, uint32_t nseg,
                                                       uint32_t chunks, uint32_t shift, uint32_t digit_bits,
-                                                      uint32_t kb, uint32_t* __restrict__ hist) {
+                                                      uint32_t kb, Runs rn, uint32_t* __restrict__ hist) {
   __shared__ uint32_t cnt[1 << PB];
+  __shared__ RunTable rt[1];  // run mode only (unreferenced otherwise)
   const uint32_t D = 1u << digit_bits;
   const uint64_t dmask = D - 1;
   const uint64_t total = (uint64_t)nseg * chunks;
   for (uint64_t gi = blockIdx.x; gi < total; gi += gridDim.x) {
     const uint32_t s = (uint32_t)(gi / chunks), p = (uint32_t)(gi % chunks);
-    uint64_t a, b;
-    chunk_range(segs, s, p, chunks, &a, &b);
+    uint64_t a = 0, b = 0;
+    uint32_t nr = 0;
+    if constexpr (RUNS)
+      nr = load_runs(rn, s, p, chunks, rt[0]);
+    else
+      chunk_range(segs, s, p, chunks, &a, &b);
     for (uint32_t d = threadIdx.x; d < D; d += kBlock) cnt[d] = 0;
     lds_barrier();
+    // Records [a, b): record indices, or (run mode) indices into the
+    // chunk's run table.
+    if constexpr (RUNS) b = rt[0].pre[kMaxChunkRuns];
+    uint32_t rcur = 0;
     for (uint64_t base = a; base < b; base += kStage) {
       uint64_t k[kPerThread];
 #pragma unroll
       for (int j = 0; j < kPerThread; ++j) {
         const uint64_t i = base + (uint64_t)j * kBlock + threadIdx.x;
-        k[j] = i < b ? rec_key(in[i], kb) : kEmptyKey;
+        k[j] = kEmptyKey;
+        if (i < b) {
+          uint64_t at = i;
+          if constexpr (RUNS) at = run_record(rt[0], nr, (uint32_t)i, rcur);
+          k[j] = rec_key(in[at], kb);
+        }
       }
 #pragma unroll
       for (int j = 0; j < kPerThread; ++j)
@@ -710,47 +803,64 @@ __global__ __launch_bounds__(kBlock) void k_part_hist(const Rec<L>* __restrict__
 // scanned, the records are placed in LDS already sorted by digit, and the
 // workgroup writes them out linearly -- consecutive threads store consecutive
 // addresses of one digit's run.
-template <int L>
-constexpr int part_stage() { return L == kPackedRecords ? 2560 : kStage; }
+// (Run mode computes record addresses per load: the smaller stage keeps its
+// registers within four waves per SIMD.)
+template <int L, bool RUNS>
+constexpr int part_stage() { return L == kPackedRecords && !RUNS ? 2560 : kStage; }
 
-template <int L, int PB>
+template <int L, int PB, bool RUNS>
 struct PartStage {
-  Rec<L> rec[part_stage<L>()];
-  std::conditional_t<(PB <= 8), uint8_t, uint16_t> digit[part_stage<L>()];
+  Rec<L> rec[part_stage<L, RUNS>()];
+  std::conditional_t<(PB <= 8), uint8_t, uint16_t> digit[part_stage<L, RUNS>()];
   uint32_t count[1 << PB];
   uint32_t start[1 << PB];
   uint64_t gpos[1 << PB];
   uint32_t wave_tot[kWaves];
 };
 
-template <int L, int PB>
+template <int L, int PB, bool RUNS>
 __global__ __launch_bounds__(kBlock, (L == kPackedRecords && PB <= 8 ? 4 : 1)) void k_part_scatter(const Rec<L>* __restrict__ in,
                                                          Rec<L>* __restrict__ out,
                                                          const uint64_t* __restrict__ segs, uint32_t nseg,
                                                          uint32_t chunks, uint32_t shift, uint32_t digit_bits,
-                                                         uint32_t kb, const uint64_t* __restrict__ offsets,
+                                                         uint32_t kb, Runs rn, const uint64_t* __restrict__ offsets,
                                                          uint64_t cap, uint64_t* __restrict__ counters) {
-  constexpr int PS = part_stage<L>();
+  constexpr int PS = part_stage<L, RUNS>();
   constexpr int PT = PS / kBlock;
   static_assert(PS % kBlock == 0, "stage is whole rounds of the workgroup");
-  __shared__ PartStage<L, PB> st;
+  __shared__ PartStage<L, PB, RUNS> st;
+  __shared__ RunTable rt[1];  // run mode only (unreferenced otherwise)
   const uint32_t D = 1u << digit_bits;
   const uint64_t dmask = D - 1;
   const uint64_t total = (uint64_t)nseg * chunks;
   for (uint64_t gi = blockIdx.x; gi < total; gi += gridDim.x) {
     const uint32_t s = (uint32_t)(gi / chunks), p = (uint32_t)(gi % chunks);
-    uint64_t a, b;
-    chunk_range(segs, s, p, chunks, &a, &b);
+    uint64_t a = 0, b = 0;
+    uint32_t nr = 0;
+    if constexpr (RUNS)
+      nr = load_runs(rn, s, p, chunks, rt[0]);
+    else
+      chunk_range(segs, s, p, chunks, &a, &b);
     for (uint32_t d = threadIdx.x; d < D; d += kBlock) {
       st.count[d] = 0;
       st.gpos[d] = offsets[((uint64_t)s * D + d) * chunks + p];
     }
     lds_barrier();
+    // Records [a, b): record indices, or (run mode) indices into the
+    // chunk's run table.
+    if constexpr (RUNS) b = rt[0].pre[kMaxChunkRuns];
+    uint32_t rcur = 0;
+    auto record_at = [&](uint64_t i) -> uint64_t {
+      if constexpr (RUNS)
+        return run_record(rt[0], nr, (uint32_t)i, rcur);
+      else
+        return i;
+    };
     Rec<L> nx[PT];
 #pragma unroll
     for (int j = 0; j < PT; ++j) {
       const uint64_t i = a + (uint64_t)j * kBlock + threadIdx.x;
-      if (i < b) nx[j] = in[i];
+      if (i < b) nx[j] = in[record_at(i)];
     }
     for (uint64_t base = a; base < b; base += PS) {
       const uint32_t nrec = (b - base < (uint64_t)PS) ? (uint32_t)(b - base) : (uint32_t)PS;
@@ -762,7 +872,7 @@ __global__ __launch_bounds__(kBlock, (L == kPackedRecords && PB <= 8 ? 4 : 1)) v
 #pragma unroll
         for (int j = 0; j < PT; ++j) {
           const uint64_t i = nb + (uint64_t)j * kBlock + threadIdx.x;
-          if (i < b) nx[j] = in[i];
+          if (i < b) nx[j] = in[record_at(i)];
         }
       }
       // 1. digit and rank of every staged record
@@ -803,6 +913,9 @@ __global__ __launch_bounds__(kBlock, (L == kPackedRecords && PB <= 8 ? 4 : 1)) v
       }
       lds_barrier();
     }
+    // The run table is rewritten by the next chunk: every wave must be past
+    // its reads (an empty chunk runs no stage and so no barrier above).
+    if constexpr (RUNS) lds_barrier();
   }
 }
 
@@ -1117,9 +1230,18 @@ __global__ __launch_bounds__(BT, (BT == kScatterBlock && L <= 2 && T == 2048 && 
   }
 
   // ---- flush statistics
-  if (MODE == kMergeCount) {
+  if (MODE == kMergeCount || (MODE == kMergeScatter && hist)) {
+    // COUNT: the digit histogram.  SCATTER with upper-bound offsets: the
+    // length of every run it wrote (the rest of each run's range is a gap).
     for (uint32_t d = threadIdx.x; d < D; d += BT) hist[(uint64_t)d * W + blockIdx.x] = lcount[d];
-  } else {
+  }
+  if (MODE == kMergeScatter) {
+    uint64_t wr = 0;
+    for (uint32_t d = threadIdx.x; d < D; d += BT) wr += lcount[d];
+    wr = block_sum_u64<BT / 64>(wr, scratch);
+    if (threadIdx.x == 0 && wr) atomicAdd((unsigned long long*)&counters[kWritten], (unsigned long long)wr);
+  }
+  if (MODE != kMergeCount) {
     const uint64_t u = block_sum_u64<BT / 64>(uniq, scratch);
     if (threadIdx.x == 0 && u) atomicAdd((unsigned long long*)&counters[kOutStates], (unsigned long long)u);
     if constexpr (P) {
@@ -1235,20 +1357,29 @@ uint32_t partition_chunks(uint64_t records, uint64_t nseg) {
 }
 
 hipError_t launch_part_hist(int lanes, const void* in, const uint64_t* segs, uint32_t nseg, uint32_t chunks,
-                            uint32_t shift, uint32_t digit_bits, uint32_t key_bits, uint32_t* hist,
-                            hipStream_t stream) {
+                            uint32_t shift, uint32_t digit_bits, uint32_t key_bits, const Runs* runs,
+                            uint32_t* hist, hipStream_t stream) {
   if (digit_bits == 0 || digit_bits > kMaxDigitBits || nseg == 0 || chunks == 0) return hipErrorInvalidValue;
   if (((uint64_t)nseg << digit_bits) > 0xffffffffull) return hipErrorInvalidValue;
+  if (runs && (runs->per_seg == 0 || (runs->per_seg + chunks - 1) / chunks > kMaxChunkRuns))
+    return hipErrorInvalidValue;
   const uint64_t total = (uint64_t)nseg * chunks;
   dim3 grid(total < 65536 ? (uint32_t)total : 65536u), block(kBlock);
   const bool narrow = digit_bits <= (uint32_t)kNarrowDigitBits;
+  const Runs rn = runs ? *runs : Runs{nullptr, nullptr, 0};
 #define PEPS_PH(LN)                                                                                           \
-  if (narrow)                                                                                                 \
-    hipLaunchKernelGGL((k_part_hist<LN, kNarrowDigitBits>), grid, block, 0, stream, (const Rec<LN>*)in, segs, \
-                       nseg, chunks, shift, digit_bits, key_bits, hist);                                     \
+  if (narrow && runs)                                                                                         \
+    hipLaunchKernelGGL((k_part_hist<LN, kNarrowDigitBits, true>), grid, block, 0, stream, (const Rec<LN>*)in, \
+                       segs, nseg, chunks, shift, digit_bits, key_bits, rn, hist);                           \
+  else if (narrow)                                                                                            \
+    hipLaunchKernelGGL((k_part_hist<LN, kNarrowDigitBits, false>), grid, block, 0, stream, (const Rec<LN>*)in,\
+                       segs, nseg, chunks, shift, digit_bits, key_bits, rn, hist);                           \
+  else if (runs)                                                                                              \
+    hipLaunchKernelGGL((k_part_hist<LN, kMaxDigitBits, true>), grid, block, 0, stream, (const Rec<LN>*)in,    \
+                       segs, nseg, chunks, shift, digit_bits, key_bits, rn, hist);                           \
   else                                                                                                        \
-    hipLaunchKernelGGL((k_part_hist<LN, kMaxDigitBits>), grid, block, 0, stream, (const Rec<LN>*)in, segs,    \
-                       nseg, chunks, shift, digit_bits, key_bits, hist);                                     \
+    hipLaunchKernelGGL((k_part_hist<LN, kMaxDigitBits, false>), grid, block, 0, stream, (const Rec<LN>*)in,   \
+                       segs, nseg, chunks, shift, digit_bits, key_bits, rn, hist);                           \
   break
   switch (lanes) {
     case kPackedRecords: PEPS_PH(kPackedRecords);
@@ -1264,18 +1395,28 @@ hipError_t launch_part_hist(int lanes, const void* in, const uint64_t* segs, uin
 
 hipError_t launch_part_scatter(int lanes, const void* in, void* out, const uint64_t* segs, uint32_t nseg,
                                uint32_t chunks, uint32_t shift, uint32_t digit_bits, uint32_t key_bits,
-                               const uint64_t* offsets, uint64_t cap, uint64_t* counters, hipStream_t stream) {
+                               const Runs* runs, const uint64_t* offsets, uint64_t cap, uint64_t* counters,
+                               hipStream_t stream) {
   if (digit_bits == 0 || digit_bits > kMaxDigitBits || nseg == 0 || chunks == 0) return hipErrorInvalidValue;
+  if (runs && (runs->per_seg == 0 || (runs->per_seg + chunks - 1) / chunks > kMaxChunkRuns))
+    return hipErrorInvalidValue;
   const uint64_t total = (uint64_t)nseg * chunks;
   dim3 grid(total < 65536 ? (uint32_t)total : 65536u), block(kBlock);
   const bool narrow = digit_bits <= (uint32_t)kNarrowDigitBits;
+  const Runs rn = runs ? *runs : Runs{nullptr, nullptr, 0};
 #define PEPS_PS(LN)                                                                                            \
-  if (narrow)                                                                                                  \
-    hipLaunchKernelGGL((k_part_scatter<LN, kNarrowDigitBits>), grid, block, 0, stream, (const Rec<LN>*)in,     \
-                       (Rec<LN>*)out, segs, nseg, chunks, shift, digit_bits, key_bits, offsets, cap, counters); \
+  if (narrow && runs)                                                                                          \
+    hipLaunchKernelGGL((k_part_scatter<LN, kNarrowDigitBits, true>), grid, block, 0, stream, (const Rec<LN>*)in,\
+                       (Rec<LN>*)out, segs, nseg, chunks, shift, digit_bits, key_bits, rn, offsets, cap, counters); \
+  else if (narrow)                                                                                             \
+    hipLaunchKernelGGL((k_part_scatter<LN, kNarrowDigitBits, false>), grid, block, 0, stream, (const Rec<LN>*)in,\
+                       (Rec<LN>*)out, segs, nseg, chunks, shift, digit_bits, key_bits, rn, offsets, cap, counters); \
+  else if (runs)                                                                                               \
+    hipLaunchKernelGGL((k_part_scatter<LN, kMaxDigitBits, true>), grid, block, 0, stream, (const Rec<LN>*)in,  \
+                       (Rec<LN>*)out, segs, nseg, chunks, shift, digit_bits, key_bits, rn, offsets, cap, counters); \
   else                                                                                                         \
-    hipLaunchKernelGGL((k_part_scatter<LN, kMaxDigitBits>), grid, block, 0, stream, (const Rec<LN>*)in,        \
-                       (Rec<LN>*)out, segs, nseg, chunks, shift, digit_bits, key_bits, offsets, cap, counters); \
+    hipLaunchKernelGGL((k_part_scatter<LN, kMaxDigitBits, false>), grid, block, 0, stream, (const Rec<LN>*)in, \
+                       (Rec<LN>*)out, segs, nseg, chunks, shift, digit_bits, key_bits, rn, offsets, cap, counters); \
   break
   switch (lanes) {
     case kPackedRecords: PEPS_PS(kPackedRecords);
@@ -1286,6 +1427,21 @@ hipError_t launch_part_scatter(int lanes, const void* in, void* out, const uint6
     default: return hipErrorInvalidValue;
   }
 #undef PEPS_PS
+  return hipGetLastError();
+}
+
+hipError_t launch_count_ub(int lanes, const void* in, const uint64_t* segs, uint32_t nseg, uint32_t W,
+                           const RowGeom& g, uint32_t digit_bits, uint32_t* hist, hipStream_t stream) {
+  if (nseg == 0 || W == 0 || digit_bits > kFineBits) return hipErrorInvalidValue;
+  const uint32_t per = (nseg + W - 1) / W;
+  switch (lanes) {
+    case kPackedRecords: hipLaunchKernelGGL(k_count_ub<kPackedRecords>, dim3(W), dim3(kBlock), 0, stream, (const Packed*)in, segs, nseg, per, g, digit_bits, hist); break;
+    case 1: hipLaunchKernelGGL(k_count_ub<1>, dim3(W), dim3(kBlock), 0, stream, (const Rec<1>*)in, segs, nseg, per, g, digit_bits, hist); break;
+    case 2: hipLaunchKernelGGL(k_count_ub<2>, dim3(W), dim3(kBlock), 0, stream, (const Rec<2>*)in, segs, nseg, per, g, digit_bits, hist); break;
+    case 3: hipLaunchKernelGGL(k_count_ub<3>, dim3(W), dim3(kBlock), 0, stream, (const Rec<3>*)in, segs, nseg, per, g, digit_bits, hist); break;
+    case 4: hipLaunchKernelGGL(k_count_ub<4>, dim3(W), dim3(kBlock), 0, stream, (const Rec<4>*)in, segs, nseg, per, g, digit_bits, hist); break;
+    default: return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 
