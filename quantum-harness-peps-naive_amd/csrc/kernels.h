@@ -15,9 +15,13 @@
 //   partition             : one more hash-radix level inside each bucket
 //   merge_expand COUNT    : per bucket, LDS-merge keys, count successors of
 //                           the merged states per next-row level-1 digit
+//                           (large rows: count_ub, every record's successors,
+//                           an upper bound, without the merge)
 //   scan                  : digit-major exclusive scan -> write offsets
-//   merge_expand SCATTER  : merge with residues, layer statistics, write the
+//   merge_expand SCATTER  : merge with weights, layer statistics, write the
 //                           successors into the next row's level-1 buckets
+//                           (after count_ub: runs with gaps, which the first
+//                           partition level reads in run mode)
 //   merge_expand LAST     : last row: merge, statistics, final v1/v2 sum
 #pragma once
 
