@@ -322,6 +322,7 @@ typedef struct {
 typedef struct {
   uint64_t submitted, completed, failed, pending;
   uint64_t launches, tasks;
+  uint64_t retries;      /* launches that exceeded the HBM budget and were re-run in halves */
   double busy_s;         /* worker time inside engine launches */
   double uptime_s;
 } peps_server_stats;

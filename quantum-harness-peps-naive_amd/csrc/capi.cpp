@@ -226,7 +226,12 @@ int peps_count_batched(uint32_t n, uint32_t split_depth, const peps_options* opt
       ++g1;
     }
     next = g1;
-    rc = peps::run_task_range(r, n, split_depth, tasks, g0, g1, record, &e);
+    auto exceeded = [&](size_t a, size_t b) {
+      double psum = 0;
+      for (size_t i = a; i < b; ++i) psum += proxy[i];
+      proj.exceeded(n, split_depth, psum, cap / 0.8);
+    };
+    rc = peps::run_task_range(r, n, split_depth, tasks, g0, g1, record, &e, exceeded);
     if (rc) return fail(rc, e, err, err_cap);
     if (overflow) return fail(PEPS_E_OVERFLOW, "batched count overflow", err, err_cap);
     if (next == tasks.size() && r.graph) {

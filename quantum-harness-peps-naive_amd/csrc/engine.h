@@ -90,6 +90,7 @@ struct RunOut {
   u128 total_candidates = 0, total_matched = 0;
   double elapsed_s = 0, kernel_s = 0;
   uint64_t device_peak_bytes = 0;
+  uint64_t peak_write_bound = 0;   // most records one SCATTER output needed (>= the exact count)
   bool replayed = false;           // the run was a HIP-graph replay of a recorded plan
   uint32_t record_bytes = 0;       // candidate record size the run used
   double alloc_s = 0;              // host time spent growing device buffers
@@ -157,6 +158,7 @@ class Engine {
   template <typename T>
   int ensure_buffer(T** ptr, size_t* cap, size_t count, std::string* err);
   uint64_t arena_bytes() const;
+  uint64_t region_cap() const;     // bytes of a candidate region allocated at its final size
 
   int device_;
   std::mutex mu_;

@@ -99,12 +99,14 @@ int Engine::ensure_region(int idx, size_t bytes, std::string* err) {
     ~Tally() { *acc += std::chrono::duration<double>(Clock::now() - t).count(); }
   } tally{&alloc_s_, ta};
   // Freeing and re-allocating a region of tens of GB costs seconds, so a
-  // region that must exceed 1 GiB is allocated once at its final size, half
-  // the arena budget, and never regrown.  A row that needs more than that
-  // fails with PEPS_E_MEMORY (the batched form then splits its task group).
+  // region that must exceed 1 GiB is allocated once at its final size and
+  // never regrown: half of the arena budget less a reserve for the
+  // histogram, offset and segment buffers, so that both regions fit.  A row
+  // that needs more than that fails with PEPS_E_MEMORY (the batched form and
+  // the server then split their task group).
   size_t want = std::max<size_t>(bytes + bytes / 4, (size_t)1 << 20);
   if (bytes > ((size_t)1 << 30)) {
-    want = (size_t)(budget_ / 2);
+    want = (size_t)region_cap();
     if (bytes > want) {
       *err = "HBM budget exceeded: a row needs " + std::to_string(bytes) + " bytes per candidate buffer, " +
              "the arena allows " + std::to_string(want);
@@ -140,14 +142,24 @@ int Engine::ensure_region(int idx, size_t bytes, std::string* err) {
   return 0;
 }
 
+uint64_t Engine::region_cap() const { return (budget_ - budget_ / 32) / 2; }
+
 uint64_t Engine::candidate_capacity(int lanes, int records) const {
-  return budget_ ? (budget_ / 2) / record_bytes(records == kRecordsLanes ? lanes : kPackedRecords) : 0;
+  return budget_ ? region_cap() / record_bytes(records == kRecordsLanes ? lanes : kPackedRecords) : 0;
 }
 
 template <typename T>
 int Engine::ensure_buffer(T** ptr, size_t* cap, size_t count, std::string* err) {
   if (*cap >= count) return 0;
-  size_t want = std::max<size_t>(count + count / 4, 1024);
+  struct Tally {
+    double* acc;
+    Clock::time_point t;
+    ~Tally() { *acc += std::chrono::duration<double>(Clock::now() - t).count(); }
+  } tally{&alloc_s_, Clock::now()};
+  // Every growth synchronises, frees and reallocates (and invalidates
+  // captured graphs); doubling keeps the number of growths logarithmic in
+  // the largest launch seen.
+  size_t want = std::max<size_t>(2 * count, 1024);
   if (*ptr) {
     HIP_TRY(hipStreamSynchronize(stream_));
     HIP_TRY(hipFree(*ptr));
@@ -662,6 +674,12 @@ int Engine::pipeline(const RunSpec& spec, Plan* plan, bool eager, RunOut* out, s
       d.push_back(dk);
       used += dk;
     }
+    // A trailing 1-bit level would read and write every record once more
+    // for one bit: widen the level before it instead.
+    if (d.size() >= 2 && d.back() == 1 && d[d.size() - 2] < kMaxDigitBits) {
+      d.pop_back();
+      d.back() += 1;
+    }
     return d;
   };
   int rc = 0;
@@ -704,6 +722,7 @@ int Engine::pipeline(const RunSpec& spec, Plan* plan, bool eager, RunOut* out, s
     out->peak_states = S0;
     out->total_candidates = out->total_matched = 0;
     out->device_peak_bytes = 0;
+    out->peak_write_bound = 0;
     out->residues.assign(L, 0);
     HIP_TRY(hipEventRecord(ev_start_, stream_));
   }
@@ -883,6 +902,7 @@ int Engine::pipeline(const RunSpec& spec, Plan* plan, bool eager, RunOut* out, s
         } else {
           C_count = rp.C_count;
         }
+        out->peak_write_bound = std::max(out->peak_write_bound, C_count);
         if (C_count) {
           unsigned d1 = rp.d1;
           std::vector<unsigned> pd = rp.pdigits;
@@ -1049,6 +1069,8 @@ int Engine::finish(const RunSpec& spec, const Plan& plan, RunOut* out, std::stri
   out->peak_states = spec.seeds.size();
   out->total_candidates = out->total_matched = 0;
   out->residues.assign(L, 0);
+  out->peak_write_bound = 0;
+  for (const RowPlan& rp : plan.rows) out->peak_write_bound = std::max(out->peak_write_bound, rp.C_count);
   uint64_t S_in = spec.seeds.size(), C = plan.C_first;
   u128 matched = plan.matched_first;
   std::vector<uint64_t> fin(L, 0);

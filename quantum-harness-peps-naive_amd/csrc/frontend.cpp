@@ -260,11 +260,19 @@ double Projection::ratio(unsigned n, unsigned depth) const {
 }
 
 void Projection::learn(unsigned n, unsigned depth, double proxy_sum, const RunOut& ro) {
-  u128 cmax = 0;
+  // The largest candidate buffer the run needed: the exact per-row counts,
+  // or the upper bounds a SCATTER wrote into.
+  u128 cmax = ro.peak_write_bound;
   for (const auto& l : ro.layers) cmax = std::max(cmax, l.completed);
   if (proxy_sum <= 0) return;
   double& r = ratio_[{n, depth}];
   r = std::max(r, (double)cmax / proxy_sum);
+}
+
+void Projection::exceeded(unsigned n, unsigned depth, double proxy_sum, double capacity) {
+  if (proxy_sum <= 0 || capacity <= 0) return;
+  double& r = ratio_[{n, depth}];
+  r = std::max(r, capacity / proxy_sum);
 }
 
 size_t max_slices(unsigned n, uint32_t slice_group) {
@@ -309,7 +317,7 @@ int run_task_groups(const Resolved& r, unsigned n, unsigned depth, const std::ve
 int run_task_range(const Resolved& r, unsigned n, unsigned depth, const std::vector<Task>& tasks,
                    size_t g0, size_t g1,
                    const std::function<void(size_t, size_t, const RunOut&)>& done,
-                   std::string* err) {
+                   std::string* err, const std::function<void(size_t, size_t)>& exceeded) {
   std::vector<std::pair<size_t, size_t>> work{{g0, g1}};
   while (!work.empty()) {
     const auto [a, b] = work.back();
@@ -319,6 +327,7 @@ int run_task_range(const Resolved& r, unsigned n, unsigned depth, const std::vec
     std::string e;
     const int rc = run_engine(r, spec, &ro, &e);
     if (rc == PEPS_E_MEMORY && b - a > 1) {
+      if (exceeded) exceeded(a, b);
       const size_t mid = a + (b - a) / 2;
       work.emplace_back(mid, b);
       work.emplace_back(a, mid);

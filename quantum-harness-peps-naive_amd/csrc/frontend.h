@@ -91,6 +91,9 @@ class Projection {
   double proxy(unsigned n, unsigned depth, uint64_t key) const;
   double ratio(unsigned n, unsigned depth) const;  // 0: not yet calibrated
   void learn(unsigned n, unsigned depth, double proxy_sum, const RunOut& ro);
+  // A launch of tasks with this proxy sum exceeded a candidate buffer of
+  // `capacity` records: the ratio is at least capacity / proxy_sum.
+  void exceeded(unsigned n, unsigned depth, double proxy_sum, double capacity);
 
  private:
   std::map<std::pair<unsigned, unsigned>, double> ratio_;
@@ -113,9 +116,11 @@ int run_task_groups(const Resolved& r, unsigned n, unsigned depth, const std::ve
 // launch that exceeds the HBM budget is split in halves.  `done` is called
 // once per successful launch with its range and output; `slice_counts` of that
 // output are per task of the range.
+// `exceeded(a, b)` (may be empty) hears of every range that did not fit the
+// HBM budget before it is split.
 int run_task_range(const Resolved& r, unsigned n, unsigned depth, const std::vector<Task>& tasks,
                    size_t g0, size_t g1,
                    const std::function<void(size_t, size_t, const RunOut&)>& done,
-                   std::string* err);
+                   std::string* err, const std::function<void(size_t, size_t)>& exceeded = nullptr);
 
 }  // namespace peps

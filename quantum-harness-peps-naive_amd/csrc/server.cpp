@@ -92,7 +92,7 @@ struct peps_server {
   std::map<unsigned, peps::Resolved> resolved;  // per N
   peps::Projection projection;
   uint64_t next_id = 1;
-  uint64_t pending = 0, submitted = 0, completed = 0, failed = 0, launches = 0, tasks_run = 0;
+  uint64_t pending = 0, submitted = 0, completed = 0, failed = 0, launches = 0, tasks_run = 0, retries = 0;
   double busy_s = 0;
   bool stop = false;
   std::thread worker;
@@ -203,6 +203,15 @@ struct peps_server {
           std::string se;
           rc = peps::run_engine(res, spec, &ro, &se);
           if (rc == PEPS_E_MEMORY && b - a > 1) {
+            // The projection under-estimated this class: learn from the
+            // failure so the next launches are admitted smaller.
+            double psum = 0;
+            for (size_t i = a; i < b; ++i) psum += bproxy[i];
+            {
+              std::lock_guard<std::mutex> g(mu);
+              projection.exceeded(n, depth, psum, cap / 0.8);
+              ++retries;
+            }
             const size_t mid = a + (b - a) / 2;
             work.emplace_back(mid, b);
             work.emplace_back(a, mid);
@@ -400,6 +409,7 @@ int peps_server_stats_get(peps_server* s, peps_server_stats* out) {
   out->pending = s->pending;
   out->launches = s->launches;
   out->tasks = s->tasks_run;
+  out->retries = s->retries;
   out->busy_s = s->busy_s;
   out->uptime_s = s->now();
   return PEPS_OK;

@@ -168,6 +168,7 @@ class ServerStats(ctypes.Structure):
         ("pending", ctypes.c_uint64),
         ("launches", ctypes.c_uint64),
         ("tasks", ctypes.c_uint64),
+        ("retries", ctypes.c_uint64),
         ("busy_s", ctypes.c_double),
         ("uptime_s", ctypes.c_double),
     ]

@@ -124,9 +124,10 @@ def test_cli_grammar_and_errors():
 def test_ctypes_layouts_match_the_c_header(tmp_path):
     """Every field offset of the ctypes mirrors equals the C compiler's
     offsetof for the same struct in include/peps_mi355x.h."""
-    from peps_mi355x.abi import ContractionResult, LayerMetric, Options
+    from peps_mi355x.abi import ContractionResult, LayerMetric, Options, RequestResult, ServerStats
     structs = {"peps_options": Options, "peps_contraction_result": ContractionResult,
-               "peps_layer_metric": LayerMetric}
+               "peps_layer_metric": LayerMetric, "peps_server_stats": ServerStats,
+               "peps_request_result": RequestResult}
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "peps_mi355x.h"', "int main(void) {"]
     for cname, cls in structs.items():
         lines.append(f'  printf("{cname} sizeof %zu\\n", sizeof({cname}));')
