@@ -76,6 +76,35 @@ __device__ __forceinline__ uint64_t rec_key(const Rec<L>& r, uint32_t kb) {
     return r.key;
 }
 
+// Streaming (non-temporal) loads for records that a pass reads once: they
+// leave L2 capacity to the partly written lines of the scattered stores.
+template <int L>
+__device__ __forceinline__ Rec<L> load_stream(const Rec<L>* p) {
+  Rec<L> r;
+  if constexpr (L == kPackedRecords) {
+#pragma unroll
+    for (int j = 0; j < 3; ++j) r.w[j] = __builtin_nontemporal_load(&p->w[j]);
+  } else {
+    r.key = __builtin_nontemporal_load(&p->key);
+#pragma unroll
+    for (int l = 0; l < L; ++l) r.r[l] = __builtin_nontemporal_load(&p->r[l]);
+  }
+  return r;
+}
+
+template <int L>
+__device__ __forceinline__ uint64_t load_key_stream(const Rec<L>* p, uint32_t kb) {
+  if constexpr (L == kPackedRecords) {
+    Packed r;
+    r.w[0] = __builtin_nontemporal_load(&p->w[0]);
+    r.w[1] = __builtin_nontemporal_load(&p->w[1]);
+    r.w[2] = 0;
+    return packed_key(r, kb);
+  } else {
+    return __builtin_nontemporal_load(&p->key);
+  }
+}
+
 // Residue of an exact 128-bit sum (hi:lo) modulo p < 2^32.
 __device__ __forceinline__ uint64_t mod_u128(uint64_t hi, uint64_t lo, uint32_t p) {
   const uint64_t t64 = (~0ull % p + 1) % p;  // 2^64 mod p
@@ -340,7 +369,7 @@ __global__ __launch_bounds__(kBlock) void k_count_ub(const Rec<L>* __restrict__ 
   if (b0 < b1) {
     const uint64_t lo = segs[b0], hi = segs[b1];
     for (uint64_t i = lo + threadIdx.x; i < hi; i += kBlock) {
-      const uint64_t key = rec_key(in[i], g.key_bits);
+      const uint64_t key = load_key_stream(in + i, g.key_bits);
       const Board bs = unpack(key, g);
       uint32_t free = legal_positions(bs, g);
       while (free) {
@@ -785,7 +814,7 @@ __global__ __launch_bounds__(kBlock) void k_part_hist(const Rec<L>* __restrict__
         if (i < b) {
           uint64_t at = i;
           if constexpr (RUNS) at = run_record(rt[0], nr, (uint32_t)i, rcur);
-          k[j] = rec_key(in[at], kb);
+          k[j] = load_key_stream(in + at, kb);
         }
       }
 #pragma unroll
@@ -860,7 +889,7 @@ __global__ __launch_bounds__(kBlock, (L == kPackedRecords && PB <= 8 ? 4 : 1)) v
 #pragma unroll
     for (int j = 0; j < PT; ++j) {
       const uint64_t i = a + (uint64_t)j * kBlock + threadIdx.x;
-      if (i < b) nx[j] = in[record_at(i)];
+      if (i < b) nx[j] = load_stream(in + record_at(i));
     }
     for (uint64_t base = a; base < b; base += PS) {
       const uint32_t nrec = (b - base < (uint64_t)PS) ? (uint32_t)(b - base) : (uint32_t)PS;
@@ -872,7 +901,7 @@ __global__ __launch_bounds__(kBlock, (L == kPackedRecords && PB <= 8 ? 4 : 1)) v
 #pragma unroll
         for (int j = 0; j < PT; ++j) {
           const uint64_t i = nb + (uint64_t)j * kBlock + threadIdx.x;
-          if (i < b) nx[j] = in[record_at(i)];
+          if (i < b) nx[j] = load_stream(in + record_at(i));
         }
       }
       // 1. digit and rank of every staged record
@@ -1008,7 +1037,7 @@ __global__ __launch_bounds__(BT, (BT == kScatterBlock && L <= 2 && T == 2048 && 
         } else if constexpr (MODE == kMergeCount) {
           buf[k].key = in[i].key;
         } else {
-          buf[k] = in[i];
+          buf[k] = load_stream(in + i);
         }
       }
     }
