@@ -169,7 +169,8 @@ __device__ __forceinline__ uint32_t top_digit(uint64_t key, uint32_t bits) {
   return bits ? (uint32_t)(mix64(key) >> (64 - bits)) : 0u;
 }
 
-// Exclusive scan of one u32 per thread over the workgroup.
+// Exclusive scan of one u32 per thread over a workgroup of NW waves.
+template <int NW = kWaves>
 __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* total,
                                                          uint32_t* wave_tot) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -183,7 +184,7 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* t
   lds_barrier();
   uint32_t before = 0, all = 0;
 #pragma unroll
-  for (int w = 0; w < kWaves; ++w) {
+  for (int w = 0; w < NW; ++w) {
     uint32_t t = wave_tot[w];
     before += (w < wave) ? t : 0;
     all += t;
@@ -230,16 +231,18 @@ __device__ __forceinline__ uint64_t block_sum_u64(uint64_t v, uint64_t* scratch)
   return s;
 }
 
-// Exclusive scan of counts[0..D) in LDS into starts[0..D).
+// Exclusive scan of counts[0..D) in LDS into starts[0..D) by a workgroup of
+// BT threads.
+template <int BT = kBlock>
 __device__ __forceinline__ void scan_digits(const uint32_t* counts, uint32_t* starts, uint32_t D,
                                             uint32_t* wave_tot) {
-  const uint32_t per = (D + kBlock - 1) / kBlock;
+  const uint32_t per = (D + BT - 1) / BT;
   const uint32_t lo = threadIdx.x * per;
   uint32_t local = 0;
   for (uint32_t i = 0; i < per; ++i)
     if (lo + i < D) local += counts[lo + i];
   uint32_t total;
-  uint32_t run = block_exclusive_scan(local, &total, wave_tot);
+  uint32_t run = block_exclusive_scan<BT / 64>(local, &total, wave_tot);
   for (uint32_t i = 0; i < per; ++i) {
     if (lo + i < D) {
       starts[lo + i] = run;
@@ -832,31 +835,42 @@ __global__ __launch_bounds__(kBlock) void k_part_hist(const Rec<L>* __restrict__
 // scanned, the records are placed in LDS already sorted by digit, and the
 // workgroup writes them out linearly -- consecutive threads store consecutive
 // addresses of one digit's run.
-// (Run mode computes record addresses per load: the smaller stage keeps its
-// registers within four waves per SIMD.)
-template <int L, bool RUNS>
-constexpr int part_stage() { return L == kPackedRecords && !RUNS ? 2560 : kStage; }
+// The length of those runs bounds the kernel more than occupancy does: one
+// 512-thread workgroup per CU holds a stage that fills the LDS (11264 packed
+// records, 44 per 8-bit digit per flush), which moves records 1.3x faster
+// than four 256-thread workgroups with 2048- or 2560-record stages.
+constexpr int kPartBlock = 512;
+constexpr int kPartLds = 152 * 1024;  // LDS budget of one partition workgroup
+template <int L, int PB, bool RUNS>
+constexpr int part_stage() {
+  // per staged record: the record and its digit; fixed: count, start and
+  // gpos per digit, the run table and the wave totals
+  constexpr int per = (int)sizeof(Rec<L>) + (PB <= 8 ? 1 : 2);
+  constexpr int fixed = (1 << PB) * 16 + 1024;
+  return (kPartLds - fixed) / per / kPartBlock * kPartBlock;
+}
 
 template <int L, int PB, bool RUNS>
 struct PartStage {
-  Rec<L> rec[part_stage<L, RUNS>()];
-  std::conditional_t<(PB <= 8), uint8_t, uint16_t> digit[part_stage<L, RUNS>()];
+  Rec<L> rec[part_stage<L, PB, RUNS>()];
+  std::conditional_t<(PB <= 8), uint8_t, uint16_t> digit[part_stage<L, PB, RUNS>()];
   uint32_t count[1 << PB];
   uint32_t start[1 << PB];
   uint64_t gpos[1 << PB];
-  uint32_t wave_tot[kWaves];
+  uint32_t wave_tot[kPartBlock / 64];
 };
 
 template <int L, int PB, bool RUNS>
-__global__ __launch_bounds__(kBlock, (L == kPackedRecords && PB <= 8 ? 4 : 1)) void k_part_scatter(const Rec<L>* __restrict__ in,
+__global__ __launch_bounds__(kPartBlock, 1) void k_part_scatter(const Rec<L>* __restrict__ in,
                                                          Rec<L>* __restrict__ out,
                                                          const uint64_t* __restrict__ segs, uint32_t nseg,
                                                          uint32_t chunks, uint32_t shift, uint32_t digit_bits,
                                                          uint32_t kb, Runs rn, const uint64_t* __restrict__ offsets,
                                                          uint64_t cap, uint64_t* __restrict__ counters) {
-  constexpr int PS = part_stage<L, RUNS>();
-  constexpr int PT = PS / kBlock;
-  static_assert(PS % kBlock == 0, "stage is whole rounds of the workgroup");
+  constexpr int PS = part_stage<L, PB, RUNS>();
+  constexpr int BT = kPartBlock;
+  constexpr int PT = PS / BT;
+  static_assert(PS % BT == 0, "stage is whole rounds of the workgroup");
   __shared__ PartStage<L, PB, RUNS> st;
   __shared__ RunTable rt[1];  // run mode only (unreferenced otherwise)
   const uint32_t D = 1u << digit_bits;
@@ -870,7 +884,7 @@ __global__ __launch_bounds__(kBlock, (L == kPackedRecords && PB <= 8 ? 4 : 1)) v
       nr = load_runs(rn, s, p, chunks, rt[0]);
     else
       chunk_range(segs, s, p, chunks, &a, &b);
-    for (uint32_t d = threadIdx.x; d < D; d += kBlock) {
+    for (uint32_t d = threadIdx.x; d < D; d += BT) {
       st.count[d] = 0;
       st.gpos[d] = offsets[((uint64_t)s * D + d) * chunks + p];
     }
@@ -888,7 +902,7 @@ __global__ __launch_bounds__(kBlock, (L == kPackedRecords && PB <= 8 ? 4 : 1)) v
     Rec<L> nx[PT];
 #pragma unroll
     for (int j = 0; j < PT; ++j) {
-      const uint64_t i = a + (uint64_t)j * kBlock + threadIdx.x;
+      const uint64_t i = a + (uint64_t)j * BT + threadIdx.x;
       if (i < b) nx[j] = load_stream(in + record_at(i));
     }
     for (uint64_t base = a; base < b; base += PS) {
@@ -900,7 +914,7 @@ __global__ __launch_bounds__(kBlock, (L == kPackedRecords && PB <= 8 ? 4 : 1)) v
       if (nb < b) {
 #pragma unroll
         for (int j = 0; j < PT; ++j) {
-          const uint64_t i = nb + (uint64_t)j * kBlock + threadIdx.x;
+          const uint64_t i = nb + (uint64_t)j * BT + threadIdx.x;
           if (i < b) nx[j] = load_stream(in + record_at(i));
         }
       }
@@ -908,17 +922,17 @@ __global__ __launch_bounds__(kBlock, (L == kPackedRecords && PB <= 8 ? 4 : 1)) v
       uint32_t dg[PT], rk[PT];
 #pragma unroll
       for (int j = 0; j < PT; ++j) {
-        const uint32_t slot = (uint32_t)j * kBlock + threadIdx.x;
+        const uint32_t slot = (uint32_t)j * BT + threadIdx.x;
         dg[j] = (uint32_t)((mix64(rec_key(cur[j], kb)) >> shift) & dmask);
         rk[j] = 0;
         if (slot < nrec) rk[j] = atomicAdd(&st.count[dg[j]], 1u);
       }
       lds_barrier();
-      scan_digits(st.count, st.start, D, st.wave_tot);
+      scan_digits<BT>(st.count, st.start, D, st.wave_tot);
       // 2. place the records sorted by digit
 #pragma unroll
       for (int j = 0; j < PT; ++j) {
-        const uint32_t slot = (uint32_t)j * kBlock + threadIdx.x;
+        const uint32_t slot = (uint32_t)j * BT + threadIdx.x;
         if (slot < nrec) {
           const uint32_t at = st.start[dg[j]] + rk[j];
           st.rec[at] = cur[j];
@@ -927,7 +941,7 @@ __global__ __launch_bounds__(kBlock, (L == kPackedRecords && PB <= 8 ? 4 : 1)) v
       }
       lds_barrier();
       // 3. linear write-out of the digit runs
-      for (uint32_t i = threadIdx.x; i < nrec; i += kBlock) {
+      for (uint32_t i = threadIdx.x; i < nrec; i += BT) {
         const uint32_t d = st.digit[i];
         const uint64_t pos = st.gpos[d] + (i - st.start[d]);
         if (pos < cap)
@@ -936,7 +950,7 @@ __global__ __launch_bounds__(kBlock, (L == kPackedRecords && PB <= 8 ? 4 : 1)) v
           atomicOr((unsigned long long*)&counters[kError], 2ull);
       }
       lds_barrier();
-      for (uint32_t d = threadIdx.x; d < D; d += kBlock) {
+      for (uint32_t d = threadIdx.x; d < D; d += BT) {
         st.gpos[d] += st.count[d];
         st.count[d] = 0;
       }
@@ -1430,21 +1444,21 @@ hipError_t launch_part_scatter(int lanes, const void* in, void* out, const uint6
   if (runs && (runs->per_seg == 0 || (runs->per_seg + chunks - 1) / chunks > kMaxChunkRuns))
     return hipErrorInvalidValue;
   const uint64_t total = (uint64_t)nseg * chunks;
-  dim3 grid(total < 65536 ? (uint32_t)total : 65536u), block(kBlock);
+  dim3 grid(total < 65536 ? (uint32_t)total : 65536u);
   const bool narrow = digit_bits <= (uint32_t)kNarrowDigitBits;
   const Runs rn = runs ? *runs : Runs{nullptr, nullptr, 0};
 #define PEPS_PS(LN)                                                                                            \
   if (narrow && runs)                                                                                          \
-    hipLaunchKernelGGL((k_part_scatter<LN, kNarrowDigitBits, true>), grid, block, 0, stream, (const Rec<LN>*)in,\
+    hipLaunchKernelGGL((k_part_scatter<LN, kNarrowDigitBits, true>), grid, dim3(kPartBlock), 0, stream, (const Rec<LN>*)in,\
                        (Rec<LN>*)out, segs, nseg, chunks, shift, digit_bits, key_bits, rn, offsets, cap, counters); \
   else if (narrow)                                                                                             \
-    hipLaunchKernelGGL((k_part_scatter<LN, kNarrowDigitBits, false>), grid, block, 0, stream, (const Rec<LN>*)in,\
+    hipLaunchKernelGGL((k_part_scatter<LN, kNarrowDigitBits, false>), grid, dim3(kPartBlock), 0, stream, (const Rec<LN>*)in,\
                        (Rec<LN>*)out, segs, nseg, chunks, shift, digit_bits, key_bits, rn, offsets, cap, counters); \
   else if (runs)                                                                                               \
-    hipLaunchKernelGGL((k_part_scatter<LN, kMaxDigitBits, true>), grid, block, 0, stream, (const Rec<LN>*)in,  \
+    hipLaunchKernelGGL((k_part_scatter<LN, kMaxDigitBits, true>), grid, dim3(kPartBlock), 0, stream, (const Rec<LN>*)in,  \
                        (Rec<LN>*)out, segs, nseg, chunks, shift, digit_bits, key_bits, rn, offsets, cap, counters); \
   else                                                                                                         \
-    hipLaunchKernelGGL((k_part_scatter<LN, kMaxDigitBits, false>), grid, block, 0, stream, (const Rec<LN>*)in, \
+    hipLaunchKernelGGL((k_part_scatter<LN, kMaxDigitBits, false>), grid, dim3(kPartBlock), 0, stream, (const Rec<LN>*)in, \
                        (Rec<LN>*)out, segs, nseg, chunks, shift, digit_bits, key_bits, rn, offsets, cap, counters); \
   break
   switch (lanes) {
