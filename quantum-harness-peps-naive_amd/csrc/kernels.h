@@ -146,7 +146,7 @@ hipError_t launch_scan(const uint32_t* hist, uint32_t D, uint32_t W, uint32_t gr
 // processed as `chunks` ranges (partition_chunks); hist and offsets need
 // nseg * 2^digit_bits * chunks entries, partial that / 4096 + 2.  A write at
 // or beyond `cap` sets bit 1 of counters[kError].
-constexpr uint64_t kPartChunk = 32768;     // target records per partition workgroup
+constexpr uint64_t kPartChunk = 65536;     // target records per partition workgroup
 constexpr uint64_t kMaxPartChunks = 4096;
 uint32_t partition_chunks(uint64_t records, uint64_t nseg);
 // Run mode of a partition level (its input is the SCATTER output with
