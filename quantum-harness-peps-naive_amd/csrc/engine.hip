@@ -674,11 +674,21 @@ int Engine::pipeline(const RunSpec& spec, Plan* plan, bool eager, RunOut* out, s
       d.push_back(dk);
       used += dk;
     }
-    // A trailing 1-bit level would read and write every record once more
-    // for one bit: widen the level before it instead.
-    if (d.size() >= 2 && d.back() == 1 && d[d.size() - 2] < kMaxDigitBits) {
-      d.pop_back();
-      d.back() += 1;
+    // A short trailing level would read and write every record once more
+    // for a few bits: spread them over the levels before it instead, one
+    // more bit each (from the last), when those levels can take them all.
+    const unsigned cap = std::min<unsigned>(kMaxDigitBits, rbits + 1);
+    if (d.size() >= 2) {
+      unsigned rest = d.back(), room = 0;
+      for (size_t i = 0; i + 1 < d.size(); ++i) room += d[i] < cap ? 1u : 0u;
+      if (rest <= room) {
+        d.pop_back();
+        for (size_t i = d.size(); i-- > 0 && rest;)
+          if (d[i] < cap) {
+            d[i] += 1;
+            rest -= 1;
+          }
+      }
     }
     return d;
   };

@@ -35,6 +35,7 @@ constexpr uint64_t kEmptyKey = ~0ull;
 constexpr uint32_t kFineBits = 10;       // histogram resolution of the COUNT pass
 constexpr uint32_t kMaxDigitBits = 11;   // widest partition digit
 constexpr int kNarrowDigitBits = 8;      // SCATTER digits up to this width use smaller LDS arrays
+constexpr int kMidDigitBits = 9;         // partition digits of 9 bits get their own LDS sizing
 
 // Geometry of one row step (uniform across the launch).  Board masks are
 // 32-bit (N <= 21).  The occupied C entry's binary legs (src/lib.rs:425-485)

@@ -1409,6 +1409,7 @@ hipError_t launch_part_hist(int lanes, const void* in, const uint64_t* segs, uin
   const uint64_t total = (uint64_t)nseg * chunks;
   dim3 grid(total < 65536 ? (uint32_t)total : 65536u), block(kBlock);
   const bool narrow = digit_bits <= (uint32_t)kNarrowDigitBits;
+  const bool mid = digit_bits <= (uint32_t)kMidDigitBits;
   const Runs rn = runs ? *runs : Runs{nullptr, nullptr, 0};
 #define PEPS_PH(LN)                                                                                           \
   if (narrow && runs)                                                                                         \
@@ -1416,6 +1417,12 @@ hipError_t launch_part_hist(int lanes, const void* in, const uint64_t* segs, uin
                        segs, nseg, chunks, shift, digit_bits, key_bits, rn, hist);                           \
   else if (narrow)                                                                                            \
     hipLaunchKernelGGL((k_part_hist<LN, kNarrowDigitBits, false>), grid, block, 0, stream, (const Rec<LN>*)in,\
+                       segs, nseg, chunks, shift, digit_bits, key_bits, rn, hist);                           \
+  else if (mid && runs)                                                                                       \
+    hipLaunchKernelGGL((k_part_hist<LN, kMidDigitBits, true>), grid, block, 0, stream, (const Rec<LN>*)in,    \
+                       segs, nseg, chunks, shift, digit_bits, key_bits, rn, hist);                           \
+  else if (mid)                                                                                               \
+    hipLaunchKernelGGL((k_part_hist<LN, kMidDigitBits, false>), grid, block, 0, stream, (const Rec<LN>*)in,   \
                        segs, nseg, chunks, shift, digit_bits, key_bits, rn, hist);                           \
   else if (runs)                                                                                              \
     hipLaunchKernelGGL((k_part_hist<LN, kMaxDigitBits, true>), grid, block, 0, stream, (const Rec<LN>*)in,    \
@@ -1446,6 +1453,7 @@ hipError_t launch_part_scatter(int lanes, const void* in, void* out, const uint6
   const uint64_t total = (uint64_t)nseg * chunks;
   dim3 grid(total < 65536 ? (uint32_t)total : 65536u);
   const bool narrow = digit_bits <= (uint32_t)kNarrowDigitBits;
+  const bool mid = digit_bits <= (uint32_t)kMidDigitBits;
   const Runs rn = runs ? *runs : Runs{nullptr, nullptr, 0};
 #define PEPS_PS(LN)                                                                                            \
   if (narrow && runs)                                                                                          \
@@ -1453,6 +1461,12 @@ hipError_t launch_part_scatter(int lanes, const void* in, void* out, const uint6
                        (Rec<LN>*)out, segs, nseg, chunks, shift, digit_bits, key_bits, rn, offsets, cap, counters); \
   else if (narrow)                                                                                             \
     hipLaunchKernelGGL((k_part_scatter<LN, kNarrowDigitBits, false>), grid, dim3(kPartBlock), 0, stream, (const Rec<LN>*)in,\
+                       (Rec<LN>*)out, segs, nseg, chunks, shift, digit_bits, key_bits, rn, offsets, cap, counters); \
+  else if (mid && runs)                                                                                        \
+    hipLaunchKernelGGL((k_part_scatter<LN, kMidDigitBits, true>), grid, dim3(kPartBlock), 0, stream, (const Rec<LN>*)in,\
+                       (Rec<LN>*)out, segs, nseg, chunks, shift, digit_bits, key_bits, rn, offsets, cap, counters); \
+  else if (mid)                                                                                                \
+    hipLaunchKernelGGL((k_part_scatter<LN, kMidDigitBits, false>), grid, dim3(kPartBlock), 0, stream, (const Rec<LN>*)in,\
                        (Rec<LN>*)out, segs, nseg, chunks, shift, digit_bits, key_bits, rn, offsets, cap, counters); \
   else if (runs)                                                                                               \
     hipLaunchKernelGGL((k_part_scatter<LN, kMaxDigitBits, true>), grid, dim3(kPartBlock), 0, stream, (const Rec<LN>*)in,  \

@@ -22,7 +22,7 @@ from . import api
 DEFAULT_GRID = {
     "radix_bits": (4, 8, 11),
     "tile_states": (512, 1024, 2048),
-    "expand_bits": (6, 7, 8),
+    "expand_bits": (5, 6, 7),
     "merge_waves": (1, 4),
 }
 BATCHED_GRID = {"slice_group": (0, 8, 32)}
