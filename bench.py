@@ -36,8 +36,10 @@ ROCPROF_NAMES = {
     "merge_scatter": "k_merge_expand<{L}, 1, {T}, 8, 512>",
     "merge_last": "k_merge_expand<{L}, 2, {T}, 10, 256>",
     "count_ub": "k_count_ub<{L}>",
-    "part_hist": "k_part_hist<{L}, 8, false>",
-    "part_hist_runs": "k_part_hist<{L}, 8, true>",
+    "part_hist": "k_part_hist<{L}, 8, false, false>",
+    "part_hist_runs": "k_part_hist<{L}, 8, true, false>",
+    "part_hist_ub": "k_part_hist<{L}, 8, false, true>",
+    "part_hist_runs_ub": "k_part_hist<{L}, 8, true, true>",
     "part_scatter": "k_part_scatter<{L}, 8, false>",
     "part_scatter_runs": "k_part_scatter<{L}, 8, true>",
 }

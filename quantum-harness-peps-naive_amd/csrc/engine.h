@@ -118,7 +118,7 @@ struct Plan {
   std::vector<RowPlan> rows;       // rows [first_row, n)
   size_t region_need = 0, hist_need = 0, offsets_need = 0, partial_need = 0;
   size_t segs_need[2] = {0, 0};
-  size_t run_start_need = 0, run_len_need = 0;
+  size_t run_start_need = 0, run_len_need = 0, ubh_need = 0;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   uint64_t generation = 0;         // buffer generation the graph was captured against
@@ -207,6 +207,8 @@ class Engine {
   size_t run_start_cap_ = 0;
   uint32_t* run_len_ = nullptr;        // upper-bound rows: SCATTER run lengths [D1][W]
   size_t run_len_cap_ = 0;
+  uint64_t* ubh_ = nullptr;            // upper-bound COUNT of the last partition level, u32 [D][W]
+  size_t ubh_cap_ = 0;
   uint64_t* counters_rows_ = nullptr;  // per-row counters, (n + 1) * kNumCounters
   size_t counters_rows_cap_ = 0;
   uint64_t* seed_ = nullptr;           // seed frontier (SoA, seed_image layout)

@@ -399,6 +399,7 @@ int Engine::run_many_once(const std::string& key, const std::vector<RunSpec>& sp
     if (!rc) rc = ensure_buffer(&segs_[1], &segs_cap_[1], plan.segs_need[1], err);
     if (!rc) rc = ensure_buffer(&run_start_, &run_start_cap_, plan.run_start_need, err);
     if (!rc) rc = ensure_buffer(&run_len_, &run_len_cap_, plan.run_len_need, err);
+    if (!rc) rc = ensure_buffer(&ubh_, &ubh_cap_, plan.ubh_need, err);
   }
   if (rc) return rc;
   // Per-group seeds, counters and slice sums, so that every group's pipeline
@@ -559,6 +560,7 @@ int Engine::run_once(const RunSpec& spec, RunOut* out, std::string* err) {
       if (!rc) rc = ensure_buffer(&segs_[1], &segs_cap_[1], plan.segs_need[1], err);
       if (!rc) rc = ensure_buffer(&run_start_, &run_start_cap_, plan.run_start_need, err);
       if (!rc) rc = ensure_buffer(&run_len_, &run_len_cap_, plan.run_len_need, err);
+      if (!rc) rc = ensure_buffer(&ubh_, &ubh_cap_, plan.ubh_need, err);
       if (!rc) rc = ensure_buffer(&counters_rows_, &counters_rows_cap_, (size_t)(n + 1) * kNumCounters, err);
       if (!rc) rc = ensure_buffer(&seed_, &seed_cap_, seed_host.size(), err);
       if (!rc && spec.per_slice)
@@ -772,17 +774,33 @@ int Engine::pipeline(const RunSpec& spec, Plan* plan, bool eager, RunOut* out, s
   uint64_t S_in = S0;
   unsigned levels = 0;
 
+  // Geometry of the merge rows (no orbit weights) and the COUNT digit width.
+  RowGeom gm = g;
+  gm.orbit_row = 0;
+  gm.orbit_mask = gm.board_mask;
+  gm.centre_bit = 0;
+  const uint32_t CB = std::min<uint32_t>(kFineBits, std::max(1u, spec.expand_bits));
+  // Set when the last partition level of the current row also did its
+  // upper-bound COUNT into ubh_ (for a merge launch of ub_W workgroups).
+  bool ub_fused = false;
+  uint32_t ub_W = 0;
+
   // Partition levels after the level-1 digit d1.  With `runs`, the first
   // level reads the SCATTER's runs (upper-bound offsets, gaps between runs)
-  // and its scan writes the exact candidate count into `exact`.
+  // and its scan writes the exact candidate count into `exact`.  With
+  // `count_ub` (the row's merge will take the upper-bound COUNT), the last
+  // level's histogram launch also counts the successors of the buckets it
+  // defines, when a segment's buckets span few enough merge workgroups.
   auto partition_to_final = [&](int from, unsigned d1, const std::vector<unsigned>& pd, uint64_t Crow,
-                                const Runs* runs, uint64_t* exact) -> int {
+                                const Runs* runs, uint64_t* exact, bool count_ub) -> int {
     cur = from;
     nseg = (uint64_t)1 << d1;
     levels = d1 ? 1 : 0;
     unsigned used = d1;
     const Runs* rn = runs;  // the first level only
-    for (unsigned dk : pd) {
+    ub_fused = false;
+    for (size_t li = 0; li < pd.size(); ++li) {
+      const unsigned dk = pd[li];
       used += dk;
       const int to = cur ^ 1;
       uint32_t chunks = partition_chunks(Crow, nseg);
@@ -795,9 +813,31 @@ int Engine::pipeline(const RunSpec& spec, Plan* plan, bool eager, RunOut* out, s
       if (!e) e = need(&partial_, &partial_cap_, &plan->partial_need, cells / 4096 + 2);
       if (e) return e;
       uint64_t* spare = row_counters(0) + n * kNumCounters;
-      LAUNCH(rn ? "part_hist_runs" : "part_hist", 8.0 * Crow,
-             launch_part_hist(LK, region_[cur], segs_[sidx], (uint32_t)nseg, chunks, 64 - used, dk, g.key_bits,
-                              rn, hist_, stream_));
+      UbCount uc{};
+      bool fuse = false;
+      if (count_ub && li + 1 == pd.size()) {
+        const uint64_t nfinal = nseg << dk;
+        const uint32_t W = (uint32_t)std::min<uint64_t>(nfinal, merge_grid);
+        const uint32_t per = (uint32_t)((nfinal + W - 1) / W);
+        if (ub_fits(dk, per, CB)) {
+          const size_t words = ((size_t)W << CB) / 2 + 1;
+          e = need(&ubh_, &ubh_cap_, &plan->ubh_need, words);
+          if (e) return e;
+          LAUNCH("fill", 8.0 * words, launch_fill(ubh_, words, 0, stream_));
+          uc = UbCount{(uint32_t*)ubh_, W, per, CB, gm};
+          fuse = true;
+          ub_W = W;
+        }
+      }
+      if (fuse)
+        LAUNCH(rn ? "part_hist_runs_ub" : "part_hist_ub", 8.0 * Crow,
+               launch_part_hist(LK, region_[cur], segs_[sidx], (uint32_t)nseg, chunks, 64 - used, dk, g.key_bits,
+                                rn, hist_, &uc, stream_));
+      else
+        LAUNCH(rn ? "part_hist_runs" : "part_hist", 8.0 * Crow,
+               launch_part_hist(LK, region_[cur], segs_[sidx], (uint32_t)nseg, chunks, 64 - used, dk, g.key_bits,
+                                rn, hist_, nullptr, stream_));
+      ub_fused = fuse;
       LAUNCH("scan", 12.0 * cells,
              launch_scan(hist_, (uint32_t)(nseg << dk), chunks, 1, offsets_, segs_[sidx ^ 1], partial_,
                          rn ? exact : spare, stream_));
@@ -843,7 +883,7 @@ int Engine::pipeline(const RunSpec& spec, Plan* plan, bool eager, RunOut* out, s
            launch_expand_scatter(LK, skeys, sres, S0, S0, g, pr, d1, (uint32_t)tiles, per_tile, offsets_,
                                  region_[0], region_bytes_[0] / rb, row_counters(0) + n * kNumCounters,
                                  stream_));
-    rc = partition_to_final(0, d1, pd, C, nullptr, nullptr);
+    rc = partition_to_final(0, d1, pd, C, nullptr, nullptr, f + 1 < n && C >= kUpperBoundRows);
     if (rc) return rc;
   }
   g.orbit_row = 0;
@@ -878,10 +918,15 @@ int Engine::pipeline(const RunSpec& spec, Plan* plan, bool eager, RunOut* out, s
         // bound per digit and workgroup, no LDS merge); the SCATTER then
         // leaves a gap after each of its runs and the first partition level
         // reads the runs.
-        const uint32_t CB = std::min<uint32_t>(kFineBits, std::max(1u, spec.expand_bits));
         const uint32_t DF = 1u << CB;
         const bool ub = eager ? C >= kUpperBoundRows : rp.upper_bound;
         rp.upper_bound = ub;
+        const bool fused = ub && ub_fused;
+        if (ub_fused && (!ub || ub_W != W)) {
+          *err = "internal: fused upper-bound COUNT does not match row " + std::to_string(row + 1);
+          return 4;
+        }
+        ub_fused = false;
         const int nsidx = sidx ^ 1;
         rc = need(&hist_, &hist_cap_, &plan->hist_need, (size_t)DF * W);
         if (!rc) rc = need(&offsets_, &offsets_cap_, &plan->offsets_need, (size_t)DF * W);
@@ -894,14 +939,17 @@ int Engine::pipeline(const RunSpec& spec, Plan* plan, bool eager, RunOut* out, s
         // (a growing buffer moves).
         uint64_t* const wpos = ub ? run_start_ : offsets_;
         uint64_t* cn = row_counters(row + 1);
-        if (ub)
+        // COUNT histogram: from the last partition level (fused) or this pass.
+        uint32_t* const chist = fused ? (uint32_t*)ubh_ : hist_;
+        if (fused) {
+        } else if (ub)
           LAUNCH("count_ub", 8.0 * C,
                  launch_count_ub(LK, region_[cur], segs_[sidx], (uint32_t)nseg, W, g, CB, hist_, stream_));
         else
           LAUNCH("merge_count", 8.0 * C,
                  launch_merge_expand(kMergeCount, LK, region_[cur], segs_[sidx], (uint32_t)nseg, W, g, pr,
                                      CB, hist_, nullptr, nullptr, 0, 0, nullptr, 0, cr, table_slots, stream_));
-        LAUNCH("scan", 12.0 * DF * W, launch_scan(hist_, DF, W, 1, wpos, segs_[nsidx], partial_, cn, stream_));
+        LAUNCH("scan", 12.0 * DF * W, launch_scan(chist, DF, W, 1, wpos, segs_[nsidx], partial_, cn, stream_));
         uint64_t C_count = 0;  // candidates the SCATTER may write (exact, or the upper bound)
         if (eager) {
           HIP_TRY(hipMemcpyAsync(h_counters_, cn, sizeof(uint64_t) * kNumCounters, hipMemcpyDeviceToHost,
@@ -929,7 +977,7 @@ int Engine::pipeline(const RunSpec& spec, Plan* plan, bool eager, RunOut* out, s
           }
           if (d1 < CB)
             LAUNCH("scan", 12.0 * DF * W,
-                   launch_scan(hist_, 1u << d1, W, 1u << (CB - d1), wpos, segs_[nsidx], partial_, cn, stream_));
+                   launch_scan(chist, 1u << d1, W, 1u << (CB - d1), wpos, segs_[nsidx], partial_, cn, stream_));
           const int to = cur ^ 1;
           rc = grow(to, C_count * rb);
           if (rc) return rc;
@@ -958,7 +1006,8 @@ int Engine::pipeline(const RunSpec& spec, Plan* plan, bool eager, RunOut* out, s
           }
           rp.C_next = C_next;
           const Runs runs{run_start_, run_len_, W};
-          rc = partition_to_final(to, d1, pd, C_next, ub ? &runs : nullptr, cn);
+          rc = partition_to_final(to, d1, pd, C_next, ub ? &runs : nullptr, cn,
+                                  row + 2 < n && C_next >= kUpperBoundRows);
           if (rc) return rc;
         } else {
           C_next = 0;
@@ -1333,7 +1382,7 @@ int Engine::materialized(const RunSpec& spec, RunOut* out, std::string* err) {
         if (rc) return rc;
         LAUNCH("part_hist", 8.0 * C,
                launch_part_hist(L, region_[cur], segs_[sidx], (uint32_t)nseg, chunks, 64 - used, dk, 0, nullptr,
-                                hist_, stream_));
+                                hist_, nullptr, stream_));
         LAUNCH("scan", 12.0 * cells,
                launch_scan(hist_, (uint32_t)(nseg << dk), chunks, 1, offsets_, segs_[sidx ^ 1], partial_, spare,
                            stream_));

@@ -161,13 +161,33 @@ struct Runs {
   const uint32_t* len;
   uint32_t per_seg;
 };
+// Upper-bound COUNT folded into the last partition level of a row (the
+// histogram launch): the successor digits (`bits` wide) of every record,
+// counted per merge workgroup of a W-workgroup merge launch that gives
+// workgroup w the final buckets [w * per, (w + 1) * per), added into
+// hist[d * W + w] (zeroed by the caller).  A workgroup counts one segment
+// at a time in kUbSlots LDS counters, one per (workgroup, digit) pair the
+// segment can reach: ub_fits(digit_bits, per, bits) must hold.
+constexpr uint32_t kUbSlots = 1024;
+struct UbCount {
+  uint32_t* hist;
+  uint32_t W, per, bits;
+  RowGeom g;
+};
+// Most merge workgroups whose buckets one segment's 2^digit_bits final
+// buckets can touch.
+inline uint32_t ub_span(uint32_t digit_bits, uint32_t per) { return ((1u << digit_bits) - 1) / per + 2; }
+inline bool ub_fits(uint32_t digit_bits, uint32_t per, uint32_t bits) {
+  return per > 0 && bits >= 1 && bits <= 10 && ((uint64_t)ub_span(digit_bits, per) << bits) <= kUbSlots;
+}
 // The partition is three launches: hist, launch_scan(hist, nseg << digit_bits,
 // chunks, 1, offsets, subsegs, partial, counters), scatter.
 // key_bits: the packed records' key width (ignored for lane records).  runs:
-// run mode (segs unused), or null for contiguous segments.
+// run mode (segs unused), or null for contiguous segments.  ub: also the
+// upper-bound COUNT of the buckets this level writes (or null).
 hipError_t launch_part_hist(int lanes, const void* in, const uint64_t* segs, uint32_t nseg, uint32_t chunks,
                             uint32_t shift, uint32_t digit_bits, uint32_t key_bits, const Runs* runs,
-                            uint32_t* hist, hipStream_t stream);
+                            uint32_t* hist, const UbCount* ub, hipStream_t stream);
 hipError_t launch_part_scatter(int lanes, const void* in, void* out, const uint64_t* segs, uint32_t nseg,
                                uint32_t chunks, uint32_t shift, uint32_t digit_bits, uint32_t key_bits,
                                const Runs* runs, const uint64_t* offsets, uint64_t cap, uint64_t* counters,

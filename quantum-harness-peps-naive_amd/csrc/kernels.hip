@@ -784,13 +784,23 @@ __device__ __forceinline__ uint64_t run_record(const RunTable& t, uint32_t nr, u
   return t.start[r] + (j - t.pre[r]);
 }
 
-template <int L, int PB, bool RUNS>
+// UB: the last partition level of a row also does the upper-bound COUNT of
+// k_count_ub.  A record of segment s with digit d lands in final bucket
+// s * D + d, which merge workgroup (s * D + d) / per owns, so its successors'
+// top digits are counted here per (digit, workgroup) instead of by a second
+// read of the buckets.  The (workgroup, digit) pairs a segment can reach fit
+// the kUbSlots LDS counters (checked by the launcher); each chunk's counts
+// are added into uc.hist[digit * W + w], which the caller zeroes.
+template <int L, int PB, bool RUNS, bool UB>
 __global__ __launch_bounds__(kBlock) void k_part_hist(const Rec<L>* __restrict__ in,
                                                       const uint64_t* __restrict__ segs, uint32_t nseg,
                                                       uint32_t chunks, uint32_t shift, uint32_t digit_bits,
-                                                      uint32_t kb, Runs rn, uint32_t* __restrict__ hist) {
+                                                      uint32_t kb, Runs rn, uint32_t* __restrict__ hist,
+                                                      UbCount uc) {
   __shared__ uint32_t cnt[1 << PB];
   __shared__ RunTable rt[1];  // run mode only (unreferenced otherwise)
+  __shared__ uint32_t ul[UB ? kUbSlots : 1];
+  __shared__ uint16_t wmap[UB ? (1 << PB) : 1];
   const uint32_t D = 1u << digit_bits;
   const uint64_t dmask = D - 1;
   const uint64_t total = (uint64_t)nseg * chunks;
@@ -803,6 +813,12 @@ __global__ __launch_bounds__(kBlock) void k_part_hist(const Rec<L>* __restrict__
     else
       chunk_range(segs, s, p, chunks, &a, &b);
     for (uint32_t d = threadIdx.x; d < D; d += kBlock) cnt[d] = 0;
+    uint32_t w0 = 0;
+    if constexpr (UB) {
+      w0 = (s * D) / uc.per;
+      for (uint32_t d = threadIdx.x; d < D; d += kBlock) wmap[d] = (uint16_t)((s * D + d) / uc.per - w0);
+      for (uint32_t i = threadIdx.x; i < kUbSlots; i += kBlock) ul[i] = 0;
+    }
     lds_barrier();
     // Records [a, b): record indices, or (run mode) indices into the
     // chunk's run table.
@@ -822,10 +838,31 @@ __global__ __launch_bounds__(kBlock) void k_part_hist(const Rec<L>* __restrict__
       }
 #pragma unroll
       for (int j = 0; j < kPerThread; ++j)
-        if (k[j] != kEmptyKey) atomicAdd(&cnt[(uint32_t)((mix64(k[j]) >> shift) & dmask)], 1u);
+        if (k[j] != kEmptyKey) {
+          const uint32_t d = (uint32_t)((mix64(k[j]) >> shift) & dmask);
+          atomicAdd(&cnt[d], 1u);
+          if constexpr (UB) {
+            const uint32_t row = (uint32_t)wmap[d] << uc.bits;
+            const Board bs = unpack(k[j], uc.g);
+            uint32_t free = legal_positions(bs, uc.g);
+            while (free) {
+              const uint32_t bit = free & (~free + 1);
+              free ^= bit;
+              atomicAdd(&ul[row + top_digit(successor(k[j], bs, bit, uc.g), uc.bits)], 1u);
+            }
+          }
+        }
     }
     lds_barrier();
     for (uint32_t d = threadIdx.x; d < D; d += kBlock) hist[((uint64_t)s * D + d) * chunks + p] = cnt[d];
+    if constexpr (UB) {
+      const uint32_t dfm = (1u << uc.bits) - 1;
+      for (uint32_t i = threadIdx.x; i < kUbSlots; i += kBlock) {
+        const uint32_t v = ul[i];
+        const uint32_t w = w0 + (i >> uc.bits);
+        if (v && w < uc.W) atomicAdd(&uc.hist[(uint64_t)(i & dfm) * uc.W + w], v);
+      }
+    }
     lds_barrier();
   }
 }
@@ -1401,35 +1438,41 @@ uint32_t partition_chunks(uint64_t records, uint64_t nseg) {
 
 hipError_t launch_part_hist(int lanes, const void* in, const uint64_t* segs, uint32_t nseg, uint32_t chunks,
                             uint32_t shift, uint32_t digit_bits, uint32_t key_bits, const Runs* runs,
-                            uint32_t* hist, hipStream_t stream) {
+                            uint32_t* hist, const UbCount* ub, hipStream_t stream) {
   if (digit_bits == 0 || digit_bits > kMaxDigitBits || nseg == 0 || chunks == 0) return hipErrorInvalidValue;
   if (((uint64_t)nseg << digit_bits) > 0xffffffffull) return hipErrorInvalidValue;
   if (runs && (runs->per_seg == 0 || (runs->per_seg + chunks - 1) / chunks > kMaxChunkRuns))
+    return hipErrorInvalidValue;
+  if (ub && (ub->hist == nullptr || ub->W == 0 || !ub_fits(digit_bits, ub->per, ub->bits) ||
+             (uint64_t)ub->W * ub->per < ((uint64_t)nseg << digit_bits)))
     return hipErrorInvalidValue;
   const uint64_t total = (uint64_t)nseg * chunks;
   dim3 grid(total < 65536 ? (uint32_t)total : 65536u), block(kBlock);
   const bool narrow = digit_bits <= (uint32_t)kNarrowDigitBits;
   const bool mid = digit_bits <= (uint32_t)kMidDigitBits;
   const Runs rn = runs ? *runs : Runs{nullptr, nullptr, 0};
-#define PEPS_PH(LN)                                                                                           \
-  if (narrow && runs)                                                                                         \
-    hipLaunchKernelGGL((k_part_hist<LN, kNarrowDigitBits, true>), grid, block, 0, stream, (const Rec<LN>*)in, \
-                       segs, nseg, chunks, shift, digit_bits, key_bits, rn, hist);                           \
-  else if (narrow)                                                                                            \
-    hipLaunchKernelGGL((k_part_hist<LN, kNarrowDigitBits, false>), grid, block, 0, stream, (const Rec<LN>*)in,\
-                       segs, nseg, chunks, shift, digit_bits, key_bits, rn, hist);                           \
-  else if (mid && runs)                                                                                       \
-    hipLaunchKernelGGL((k_part_hist<LN, kMidDigitBits, true>), grid, block, 0, stream, (const Rec<LN>*)in,    \
-                       segs, nseg, chunks, shift, digit_bits, key_bits, rn, hist);                           \
-  else if (mid)                                                                                               \
-    hipLaunchKernelGGL((k_part_hist<LN, kMidDigitBits, false>), grid, block, 0, stream, (const Rec<LN>*)in,   \
-                       segs, nseg, chunks, shift, digit_bits, key_bits, rn, hist);                           \
-  else if (runs)                                                                                              \
-    hipLaunchKernelGGL((k_part_hist<LN, kMaxDigitBits, true>), grid, block, 0, stream, (const Rec<LN>*)in,    \
-                       segs, nseg, chunks, shift, digit_bits, key_bits, rn, hist);                           \
-  else                                                                                                        \
-    hipLaunchKernelGGL((k_part_hist<LN, kMaxDigitBits, false>), grid, block, 0, stream, (const Rec<LN>*)in,   \
-                       segs, nseg, chunks, shift, digit_bits, key_bits, rn, hist);                           \
+  const UbCount uc = ub ? *ub : UbCount{nullptr, 0, 1, 0, RowGeom{}};
+#define PEPS_PH_ONE(LN, PB, RUNS, UB)                                                                     \
+  hipLaunchKernelGGL((k_part_hist<LN, PB, RUNS, UB>), grid, block, 0, stream, (const Rec<LN>*)in, segs, nseg, \
+                     chunks, shift, digit_bits, key_bits, rn, hist, uc)
+#define PEPS_PH_UB(LN, PB, RUNS)          \
+  if (ub)                                 \
+    PEPS_PH_ONE(LN, PB, RUNS, true);      \
+  else                                    \
+    PEPS_PH_ONE(LN, PB, RUNS, false)
+#define PEPS_PH(LN)                                     \
+  if (narrow && runs)                                   \
+    PEPS_PH_UB(LN, kNarrowDigitBits, true);             \
+  else if (narrow)                                      \
+    PEPS_PH_UB(LN, kNarrowDigitBits, false);            \
+  else if (mid && runs)                                 \
+    PEPS_PH_UB(LN, kMidDigitBits, true);                \
+  else if (mid)                                         \
+    PEPS_PH_UB(LN, kMidDigitBits, false);               \
+  else if (runs)                                        \
+    PEPS_PH_UB(LN, kMaxDigitBits, true);                \
+  else                                                  \
+    PEPS_PH_UB(LN, kMaxDigitBits, false);               \
   break
   switch (lanes) {
     case kPackedRecords: PEPS_PH(kPackedRecords);
@@ -1440,6 +1483,8 @@ hipError_t launch_part_hist(int lanes, const void* in, const uint64_t* segs, uin
     default: return hipErrorInvalidValue;
   }
 #undef PEPS_PH
+#undef PEPS_PH_UB
+#undef PEPS_PH_ONE
   return hipGetLastError();
 }
 

@@ -135,6 +135,30 @@ def test_batched_n16_and_grouped_n20_shape():
     assert b.count.value == 14772512 and tasks == len(o.prefix_tasks(16, 2))
 
 
+def test_fused_and_separate_upper_bound_count_agree():
+    # Rows of 2^20+ candidates size their SCATTER from an upper-bound COUNT.
+    # With the default 6-bit SCATTER digit the large rows' count is folded
+    # into the last partition level's histogram launch; a 10-bit digit has
+    # no room in its LDS counters, so there it stays a pass of its own.
+    def run(opts):
+        pm.profile_kernels(True)
+        try:
+            r = pm.contract_rows(17, opts)
+            ks = pm.kernel_stats()
+        finally:
+            pm.profile_kernels(False)
+        return r, ks
+
+    fused, kf = run(None)
+    sep, ksep = run(pm.PepsOptions(expand_bits=10))
+    show("fused", sorted(kf), "separate", sorted(ksep))
+    assert any(k.startswith("part_hist") and k.endswith("_ub") for k in kf)
+    assert "count_ub" in ksep
+    assert not any(k.startswith("part_hist") and k.endswith("_ub") for k in ksep)
+    assert fused.count.value == sep.count.value == o.known_count(17)
+    assert tuples(fused) == tuples(sep)
+
+
 def test_repeated_runs_identical():
     a = pm.contract_rows(12)
     b = pm.contract_rows(12)
