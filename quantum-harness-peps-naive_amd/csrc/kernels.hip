@@ -165,8 +165,17 @@ __device__ __forceinline__ uint64_t successor(uint64_t key, const Board& s, uint
   return (key & ~g.state_mask) | (uint64_t)c | ((uint64_t)dr << g.n) | ((uint64_t)dl << (2 * g.n));
 }
 
+// The top `bits` bits of mix64(key), for bits <= 29 (callers take at most
+// kFineBits).  The final xorshift by 29 leaves the product's top 29 bits as
+// they are, so only the product's high word is formed: with k = hi:lo and
+// the multiplier ch:cl, it is umulhi(lo, cl) + lo * ch + hi * cl mod 2^32.
 __device__ __forceinline__ uint32_t top_digit(uint64_t key, uint32_t bits) {
-  return bits ? (uint32_t)(mix64(key) >> (64 - bits)) : 0u;
+  if (bits == 0) return 0u;
+  const uint64_t k = key ^ (key >> 31);
+  const uint32_t lo = (uint32_t)k, hi = (uint32_t)(k >> 32);
+  constexpr uint32_t ch = 0x9e3779b9u, cl = 0x7f4a7c15u;
+  const uint32_t ph = __umulhi(lo, cl) + lo * ch + hi * cl;
+  return ph >> (32 - bits);
 }
 
 // Exclusive scan of one u32 per thread over a workgroup of NW waves.
