@@ -15,8 +15,10 @@
 //   partition             : one more hash-radix level inside each bucket
 //   merge_expand COUNT    : per bucket, LDS-merge keys, count successors of
 //                           the merged states per next-row level-1 digit
-//                           (large rows: count_ub, every record's successors,
-//                           an upper bound, without the merge)
+//                           (large rows: every record's successors, an
+//                           upper bound, without the merge -- counted by the
+//                           last partition level's histogram launch, or by
+//                           count_ub where that level cannot hold it)
 //   scan                  : digit-major exclusive scan -> write offsets
 //   merge_expand SCATTER  : merge with weights, layer statistics, write the
 //                           successors into the next row's level-1 buckets
