@@ -157,6 +157,16 @@ def test_fused_and_separate_upper_bound_count_agree():
     assert not any(k.startswith("part_hist") and k.endswith("_ub") for k in ksep)
     assert fused.count.value == sep.count.value == o.known_count(17)
     assert tuples(fused) == tuples(sep)
+    # Graph replay keeps the fused count's zeroed histogram and its buffer
+    # (plain: call 1 records the size plan; batched: calls 3+ replay).
+    for rep in range(3):
+        show("fused graph", rep)
+        g = pm.contract_rows(17, pm.PepsOptions(graph_mode=1))
+        assert tuples(g) == tuples(fused), rep
+    eager_b = pm.count_batched(17, 2, task_counts=True)
+    for rep in range(4):
+        b = pm.count_batched(17, 2, pm.PepsOptions(graph_mode=1), task_counts=True)
+        assert b[2] == eager_b[2] and b[0].count.value == o.known_count(17), rep
 
 
 def test_repeated_runs_identical():
