@@ -923,8 +923,10 @@ int Engine::pipeline(const RunSpec& spec, Plan* plan, bool eager, RunOut* out, s
         rp.upper_bound = ub;
         const bool fused = ub && ub_fused;
         if (ub_fused && (!ub || ub_W != W)) {
+          // A broken plan, not a size problem: fail rather than let the
+          // batched form or the server split the group and retry.
           *err = "internal: fused upper-bound COUNT does not match row " + std::to_string(row + 1);
-          return 4;
+          return 3;
         }
         ub_fused = false;
         const int nsidx = sidx ^ 1;
@@ -941,11 +943,10 @@ int Engine::pipeline(const RunSpec& spec, Plan* plan, bool eager, RunOut* out, s
         uint64_t* cn = row_counters(row + 1);
         // COUNT histogram: from the last partition level (fused) or this pass.
         uint32_t* const chist = fused ? (uint32_t*)ubh_ : hist_;
-        if (fused) {
-        } else if (ub)
+        if (ub && !fused)
           LAUNCH("count_ub", 8.0 * C,
                  launch_count_ub(LK, region_[cur], segs_[sidx], (uint32_t)nseg, W, g, CB, hist_, stream_));
-        else
+        else if (!ub)
           LAUNCH("merge_count", 8.0 * C,
                  launch_merge_expand(kMergeCount, LK, region_[cur], segs_[sidx], (uint32_t)nseg, W, g, pr,
                                      CB, hist_, nullptr, nullptr, 0, 0, nullptr, 0, cr, table_slots, stream_));
