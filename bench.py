@@ -39,7 +39,6 @@ ROCPROF_NAMES = {
     "part_hist": "k_part_hist<{L}, 8, false, false>",
     "part_hist_runs": "k_part_hist<{L}, 8, true, false>",
     "part_hist_ub": "k_part_hist<{L}, 8, false, true>",
-    "part_hist_runs_ub": "k_part_hist<{L}, 8, true, true>",
     "part_scatter": "k_part_scatter<{L}, 8, false>",
     "part_scatter_runs": "k_part_scatter<{L}, 8, true>",
 }

@@ -815,7 +815,10 @@ int Engine::pipeline(const RunSpec& spec, Plan* plan, bool eager, RunOut* out, s
       uint64_t* spare = row_counters(0) + n * kNumCounters;
       UbCount uc{};
       bool fuse = false;
-      if (count_ub && li + 1 == pd.size()) {
+      // (Not on a run-mode level: a row whose only level reads runs reaches
+      // about W / 2^CB + 2 merge workgroups per segment, more than the LDS
+      // counters hold at the default digit width; such rows use count_ub.)
+      if (count_ub && li + 1 == pd.size() && !rn) {
         const uint64_t nfinal = nseg << dk;
         const uint32_t W = (uint32_t)std::min<uint64_t>(nfinal, merge_grid);
         const uint32_t per = (uint32_t)((nfinal + W - 1) / W);
@@ -830,7 +833,7 @@ int Engine::pipeline(const RunSpec& spec, Plan* plan, bool eager, RunOut* out, s
         }
       }
       if (fuse)
-        LAUNCH(rn ? "part_hist_runs_ub" : "part_hist_ub", 8.0 * Crow,
+        LAUNCH("part_hist_ub", 8.0 * Crow,
                launch_part_hist(LK, region_[cur], segs_[sidx], (uint32_t)nseg, chunks, 64 - used, dk, g.key_bits,
                                 rn, hist_, &uc, stream_));
       else

@@ -164,7 +164,7 @@ struct Runs {
   uint32_t per_seg;
 };
 // Upper-bound COUNT folded into the last partition level of a row (the
-// histogram launch): the successor digits (`bits` wide) of every record,
+// histogram launch of a contiguous level; not with `runs`): the successor digits (`bits` wide) of every record,
 // counted per merge workgroup of a W-workgroup merge launch that gives
 // workgroup w the final buckets [w * per, (w + 1) * per), added into
 // hist[d * W + w] (zeroed by the caller).  A workgroup counts one segment

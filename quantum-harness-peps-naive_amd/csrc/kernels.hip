@@ -793,8 +793,8 @@ __device__ __forceinline__ uint64_t run_record(const RunTable& t, uint32_t nr, u
   return t.start[r] + (j - t.pre[r]);
 }
 
-// UB: the last partition level of a row also does the upper-bound COUNT of
-// k_count_ub.  A record of segment s with digit d lands in final bucket
+// UB (contiguous levels only, not run mode): the last partition level of a
+// row also does the upper-bound COUNT of k_count_ub.  A record of segment s with digit d lands in final bucket
 // s * D + d, which merge workgroup (s * D + d) / per owns, so its successors'
 // top digits are counted here per (digit, workgroup) instead of by a second
 // read of the buckets.  The (workgroup, digit) pairs a segment can reach fit
@@ -806,6 +806,7 @@ __global__ __launch_bounds__(kBlock) void k_part_hist(const Rec<L>* __restrict__
                                                       uint32_t chunks, uint32_t shift, uint32_t digit_bits,
                                                       uint32_t kb, Runs rn, uint32_t* __restrict__ hist,
                                                       UbCount uc) {
+  static_assert(!(RUNS && UB), "the fused COUNT runs on contiguous levels only");
   __shared__ uint32_t cnt[1 << PB];
   __shared__ RunTable rt[1];  // run mode only (unreferenced otherwise)
   __shared__ uint32_t ul[UB ? kUbSlots : 1];
@@ -1452,7 +1453,7 @@ hipError_t launch_part_hist(int lanes, const void* in, const uint64_t* segs, uin
   if (((uint64_t)nseg << digit_bits) > 0xffffffffull) return hipErrorInvalidValue;
   if (runs && (runs->per_seg == 0 || (runs->per_seg + chunks - 1) / chunks > kMaxChunkRuns))
     return hipErrorInvalidValue;
-  if (ub && (ub->hist == nullptr || ub->W == 0 || !ub_fits(digit_bits, ub->per, ub->bits) ||
+  if (ub && (runs || ub->hist == nullptr || ub->W == 0 || !ub_fits(digit_bits, ub->per, ub->bits) ||
              (uint64_t)ub->W * ub->per < ((uint64_t)nseg << digit_bits)))
     return hipErrorInvalidValue;
   const uint64_t total = (uint64_t)nseg * chunks;
@@ -1464,24 +1465,24 @@ hipError_t launch_part_hist(int lanes, const void* in, const uint64_t* segs, uin
 #define PEPS_PH_ONE(LN, PB, RUNS, UB)                                                                     \
   hipLaunchKernelGGL((k_part_hist<LN, PB, RUNS, UB>), grid, block, 0, stream, (const Rec<LN>*)in, segs, nseg, \
                      chunks, shift, digit_bits, key_bits, rn, hist, uc)
-#define PEPS_PH_UB(LN, PB, RUNS)          \
+#define PEPS_PH_UB(LN, PB)                \
   if (ub)                                 \
-    PEPS_PH_ONE(LN, PB, RUNS, true);      \
+    PEPS_PH_ONE(LN, PB, false, true);     \
   else                                    \
-    PEPS_PH_ONE(LN, PB, RUNS, false)
+    PEPS_PH_ONE(LN, PB, false, false)
 #define PEPS_PH(LN)                                     \
   if (narrow && runs)                                   \
-    PEPS_PH_UB(LN, kNarrowDigitBits, true);             \
+    PEPS_PH_ONE(LN, kNarrowDigitBits, true, false);     \
   else if (narrow)                                      \
-    PEPS_PH_UB(LN, kNarrowDigitBits, false);            \
+    PEPS_PH_UB(LN, kNarrowDigitBits);                   \
   else if (mid && runs)                                 \
-    PEPS_PH_UB(LN, kMidDigitBits, true);                \
+    PEPS_PH_ONE(LN, kMidDigitBits, true, false);        \
   else if (mid)                                         \
-    PEPS_PH_UB(LN, kMidDigitBits, false);               \
+    PEPS_PH_UB(LN, kMidDigitBits);                      \
   else if (runs)                                        \
-    PEPS_PH_UB(LN, kMaxDigitBits, true);                \
+    PEPS_PH_ONE(LN, kMaxDigitBits, true, false);        \
   else                                                  \
-    PEPS_PH_UB(LN, kMaxDigitBits, false);               \
+    PEPS_PH_UB(LN, kMaxDigitBits);                      \
   break
   switch (lanes) {
     case kPackedRecords: PEPS_PH(kPackedRecords);
