@@ -54,7 +54,11 @@ enum {
   PEPS_E_MEMORY = 4,       /* HBM budget exceeded */
   PEPS_E_UNKNOWN = 5,      /* peps_known_count: N outside the table */
   PEPS_E_BUSY = 6,         /* serving: the request queue is full */
-  PEPS_E_PENDING = 7       /* serving: the request has not finished yet */
+  PEPS_E_PENDING = 7,      /* serving: the request has not finished yet */
+  PEPS_E_INTERNAL = 8      /* an internal consistency check failed (LDS merge table
+                              overflow, a write past a buffer's planned capacity, a
+                              replayed size plan that no longer matches the device);
+                              never retried as a memory shortage */
 };
 
 /* Rust u128 as two little-endian 64-bit halves. */

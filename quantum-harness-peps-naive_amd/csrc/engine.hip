@@ -253,6 +253,23 @@ namespace {
 // Internal return code: a packed record's weight outgrew its field.
 constexpr int kRcPackedOverflow = 100;
 
+// Return code of a failed consistency check (PEPS_E_INTERNAL): a device
+// error flag or a size plan that does not match the device.  Callers that
+// split a task group and retry do so for PEPS_E_MEMORY only, which only an
+// allocation the arena refuses returns.
+constexpr int kRcInternal = 8;
+
+// The error-flag bits of a row's kError counter, in words.
+std::string error_flag_text(uint64_t bits, const std::string& where) {
+  std::string s;
+  auto add = [&](const char* t) { s += (s.empty() ? "" : "; ") + std::string(t); };
+  if (bits & 1) add("LDS merge table overflow");
+  if (bits & 2) add("a kernel write reached past its buffer's planned capacity");
+  if (bits & 8) add("sitewise partial-row limit exceeded");
+  if (bits & ~11ull) add("unknown device error flag");
+  return s + where;
+}
+
 // Rows with at least this many candidates use the upper-bound COUNT.
 constexpr uint64_t kUpperBoundRows = (uint64_t)1 << 20;
 
@@ -929,7 +946,7 @@ int Engine::pipeline(const RunSpec& spec, Plan* plan, bool eager, RunOut* out, s
           // A broken plan, not a size problem: fail rather than let the
           // batched form or the server split the group and retry.
           *err = "internal: fused upper-bound COUNT does not match row " + std::to_string(row + 1);
-          return 3;
+          return kRcInternal;
         }
         ub_fused = false;
         const int nsidx = sidx ^ 1;
@@ -1028,8 +1045,8 @@ int Engine::pipeline(const RunSpec& spec, Plan* plan, bool eager, RunOut* out, s
         SYNC_AND_RESOLVE();
         if (h_counters_[kError] & 4) return kRcPackedOverflow;
         if (h_counters_[kError]) {
-          *err = "LDS merge table overflow in row " + std::to_string(row + 1);
-          return 4;
+          *err = error_flag_text(h_counters_[kError], " in row " + std::to_string(row + 1));
+          return kRcInternal;
         }
         std::vector<uint64_t> w(L);
         for (int l = 0; l < L; ++l) {
@@ -1076,8 +1093,8 @@ int Engine::pipeline(const RunSpec& spec, Plan* plan, bool eager, RunOut* out, s
   HIP_TRY(hipMemcpy(spare.data(), row_counters(n), sizeof(uint64_t) * kNumCounters, hipMemcpyDeviceToHost));
   if (spare[kError] & 4) return kRcPackedOverflow;
   if (spare[kError]) {
-    *err = "candidate buffer capacity exceeded";
-    return 4;
+    *err = error_flag_text(spare[kError], " (candidate buffers)");
+    return kRcInternal;
   }
   // Final v1/v2 contraction: the merges summed the residues of the states
   // whose columns are all set.
@@ -1124,8 +1141,8 @@ int Engine::finish(const RunSpec& spec, const Plan& plan, RunOut* out, std::stri
   for (unsigned row = 0; row <= n; ++row)
     if (cnt[(size_t)row * kNumCounters + kError] & 4) return kRcPackedOverflow;
   if (cnt[(size_t)n * kNumCounters + kError]) {
-    *err = "candidate buffer capacity exceeded (stale size plan)";
-    return 4;
+    *err = error_flag_text(cnt[(size_t)n * kNumCounters + kError], " (candidate buffers, replayed size plan)");
+    return kRcInternal;
   }
   out->record_bytes = (uint32_t)record_bytes(packed_records(spec) ? kPackedRecords : L);
   out->layers.clear();
@@ -1147,12 +1164,12 @@ int Engine::finish(const RunSpec& spec, const Plan& plan, RunOut* out, std::stri
     lo.matched = matched;
     lo.completed = C;
     if (c[kError]) {
-      *err = "LDS merge table overflow in row " + std::to_string(row + 1);
-      return 4;
+      *err = error_flag_text(c[kError], " in row " + std::to_string(row + 1) + " (replayed size plan)");
+      return kRcInternal;
     }
     if (row + 1 < n && C && cnt[(size_t)(row + 1) * kNumCounters + kCandidates] != rp.C_next) {
       *err = "replayed size plan does not match the device counts";
-      return 4;
+      return kRcInternal;
     }
     if (C) {
       std::vector<uint64_t> w(L);
@@ -1409,14 +1426,13 @@ int Engine::materialized(const RunSpec& spec, RunOut* out, std::string* err) {
       rc = read_counters(cr);
       if (rc) return rc;
       if (h[kError]) {
-        *err = (h[kError] & 1) ? "LDS merge table overflow in row " + std::to_string(row + 1)
-                               : "frontier capacity exceeded in row " + std::to_string(row + 1);
-        return 4;
+        *err = error_flag_text(h[kError], " in row " + std::to_string(row + 1));
+        return kRcInternal;
       }
       S_out = h[kOutStates];
       if (h[kEmitted] != S_out) {
         *err = "merged state count disagrees with the states written in row " + std::to_string(row + 1);
-        return 4;
+        return kRcInternal;
       }
       std::vector<uint64_t> w(L);
       for (int l = 0; l < L; ++l) {
@@ -1467,8 +1483,8 @@ int Engine::materialized(const RunSpec& spec, RunOut* out, std::string* err) {
   std::vector<uint64_t> sp(kNumCounters);
   HIP_TRY(hipMemcpy(sp.data(), spare, sizeof(uint64_t) * kNumCounters, hipMemcpyDeviceToHost));
   if (sp[kError]) {
-    *err = "candidate buffer capacity exceeded";
-    return 4;
+    *err = error_flag_text(sp[kError], " (candidate buffers)");
+    return kRcInternal;
   }
   u128 count = 0;
   if (S) {

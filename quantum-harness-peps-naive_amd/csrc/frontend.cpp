@@ -1,6 +1,8 @@
 // frontend.cpp -- option resolution, prefix tasks and sliced launches.
 #include "frontend.h"
 
+#include "split_retry.h"
+
 #include <sys/resource.h>
 
 #include <algorithm>
@@ -318,28 +320,18 @@ int run_task_range(const Resolved& r, unsigned n, unsigned depth, const std::vec
                    size_t g0, size_t g1,
                    const std::function<void(size_t, size_t, const RunOut&)>& done,
                    std::string* err, const std::function<void(size_t, size_t)>& exceeded) {
-  std::vector<std::pair<size_t, size_t>> work{{g0, g1}};
-  while (!work.empty()) {
-    const auto [a, b] = work.back();
-    work.pop_back();
+  auto run = [&](size_t a, size_t b) -> int {
     RunSpec spec = task_spec(n, depth, tasks, a, b);
     RunOut ro;
     std::string e;
     const int rc = run_engine(r, spec, &ro, &e);
-    if (rc == PEPS_E_MEMORY && b - a > 1) {
-      if (exceeded) exceeded(a, b);
-      const size_t mid = a + (b - a) / 2;
-      work.emplace_back(mid, b);
-      work.emplace_back(a, mid);
-      continue;
-    }
-    if (rc) {
+    if (rc == PEPS_OK)
+      done(a, b, ro);
+    else
       *err = e;
-      return rc;
-    }
-    done(a, b, ro);
-  }
-  return PEPS_OK;
+    return rc;
+  };
+  return split_retry(g0, g1, run, exceeded);
 }
 
 }  // namespace peps

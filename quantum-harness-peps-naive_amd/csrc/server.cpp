@@ -25,6 +25,7 @@
 
 #include "../../include/peps_mi355x.h"
 #include "frontend.h"
+#include "split_retry.h"
 
 using peps::fail;
 using peps::u128;
@@ -193,40 +194,32 @@ struct peps_server {
       std::vector<peps::RunOut> outs;
       int rc = PEPS_OK;
       {
-        std::vector<std::pair<size_t, size_t>> work{{0, batch.size()}};
-        while (!work.empty() && rc == PEPS_OK) {
-          const auto [a, b] = work.back();
-          work.pop_back();
+        auto run = [&](size_t a, size_t b) -> int {
           std::vector<Request*> slice_owner;
           peps::RunSpec spec = request_slices_spec(n, depth, batch, owner, a, b, &slice_owner);
           peps::RunOut ro;
           std::string se;
-          rc = peps::run_engine(res, spec, &ro, &se);
-          if (rc == PEPS_E_MEMORY && b - a > 1) {
-            // The projection under-estimated this class: learn from the
-            // failure so the next launches are admitted smaller.
-            double psum = 0;
-            for (size_t i = a; i < b; ++i) psum += bproxy[i];
-            {
-              std::lock_guard<std::mutex> g(mu);
-              projection.exceeded(n, depth, psum, cap / 0.8);
-              ++retries;
-            }
-            const size_t mid = a + (b - a) / 2;
-            work.emplace_back(mid, b);
-            work.emplace_back(a, mid);
-            rc = PEPS_OK;
-            continue;
-          }
-          if (rc) {
+          const int code = peps::run_engine(res, spec, &ro, &se);
+          if (code != PEPS_OK) {
             e = se;
-            break;
+            return code;
           }
           for (size_t sl = 0; sl < slice_owner.size(); ++sl) credit.emplace_back(slice_owner[sl], ro.slice_counts[sl]);
           ranges.emplace_back(a, b);
           ro.slice_counts.clear();
           outs.push_back(std::move(ro));
-        }
+          return PEPS_OK;
+        };
+        // The projection under-estimated this class: learn from the failure
+        // so the next launches are admitted smaller.
+        auto exceeded = [&](size_t a, size_t b) {
+          double psum = 0;
+          for (size_t i = a; i < b; ++i) psum += bproxy[i];
+          std::lock_guard<std::mutex> g(mu);
+          projection.exceeded(n, depth, psum, cap / 0.8);
+          ++retries;
+        };
+        rc = peps::split_retry(0, batch.size(), run, exceeded);
       }
       const double t_end = now();
 

@@ -19,6 +19,7 @@ PEPS_E_MEMORY = 4
 PEPS_E_UNKNOWN = 5
 PEPS_E_BUSY = 6
 PEPS_E_PENDING = 7
+PEPS_E_INTERNAL = 8
 
 PEPS_POLICY_SHORTEST_FIRST = 0
 PEPS_POLICY_FIFO = 1
