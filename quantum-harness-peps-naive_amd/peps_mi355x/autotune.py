@@ -6,7 +6,8 @@ Sweeps the engine's tunables -- hash-radix digit width per partition level
 scatter writes (``expand_bits``), the merge grid (``merge_waves``) and, for
 the batched form, the slice grouping (``slice_group``) -- times each shape on
 the real workload, rejects any shape whose count differs from the known
-count, and caches the fastest per (device, library version, N, form, depth).
+count, and caches the fastest per (device, library version and build, N, form,
+depth).
 
     best, table = autotune(18)                 # plain form
     best, table = autotune(18, form="batched")
@@ -33,9 +34,22 @@ def cache_path() -> str:
                           os.path.join(os.path.expanduser("~"), ".cache", "peps_mi355x", "autotune.json"))
 
 
+def build_id() -> str:
+    """SHA-256 prefix of the loaded library's bytes: a rebuilt library (new
+    kernels, new fusion rules) never reuses a shape cached for an older one."""
+    import hashlib
+
+    h = hashlib.sha256()
+    with open(api.lib_path(), "rb") as f:
+        for block in iter(lambda: f.read(1 << 20), b""):
+            h.update(block)
+    return h.hexdigest()[:16]
+
+
 def _key(n, form, depth):
     info = api.device_info(0)
-    return f"{info['name']}|{api.version()}|n={n}|{form}|depth={depth if form == 'batched' else 0}"
+    return (f"{info['name']}|{api.version()}|build={build_id()}|n={n}|{form}"
+            f"|depth={depth if form == 'batched' else 0}")
 
 
 def _load(path):

@@ -69,3 +69,15 @@ def test_gpu_sweep_all_shapes_exact(monkeypatch, tmp_path):
     assert all(t is not None for _, t in tb)
     b, _ = pm.count_batched(13, 2, bb)
     assert b.count.value == 73712
+
+
+def test_cache_key_follows_the_library_build(tmp_path, monkeypatch):
+    # A rebuilt library must not reuse shapes cached for an older build.
+    lib = tmp_path / "libpeps_mi355x.so"
+    monkeypatch.setattr(pm.api, "lib_path", lambda: str(lib))
+    lib.write_bytes(b"build one")
+    a = at.build_id()
+    lib.write_bytes(b"build two")
+    b = at.build_id()
+    lib.write_bytes(b"build one")
+    assert a != b and at.build_id() == a and len(a) == 16

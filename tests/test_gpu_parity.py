@@ -130,7 +130,7 @@ def test_batched_matches_plain(depth):
         assert bw == pw
 
 
-def test_batched_n16_and_grouped_n20_shape():
+def test_batched_n16_two_row_tasks():
     b, tasks = pm.count_batched(16, 2)
     assert b.count.value == 14772512 and tasks == len(o.prefix_tasks(16, 2))
 
