@@ -165,4 +165,15 @@ uint64_t peak_rss_bytes();
 
 std::string u128_to_string(u128 v);
 
+// layer_census.cpp: per-depth orbit-weighted prefix counts (index r = r rows
+// placed, r = 0..n) and the unweighted per-row state counts of the default
+// row loop, by a DFS and by a breadth-first pass grouped by column set.
+std::vector<u128> depth_counts(unsigned n, bool orbits, unsigned threads);
+u128 count_prefix_parallel(unsigned n, const uint32_t* cols, unsigned k, unsigned threads);
+struct CensusRow {
+  unsigned row;
+  uint64_t input_states, matched, completed, output_states;
+};
+std::vector<CensusRow> frontier_census(unsigned n, bool orbits, unsigned threads);
+
 }  // namespace oracle

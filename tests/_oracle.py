@@ -43,6 +43,12 @@ def lib():
         L.oracle_site_tensors.argtypes = [c.POINTER(c.c_uint8), c.POINTER(c.c_uint8), c.POINTER(c.c_uint32)]
         L.oracle_row_transfer.argtypes = [c.c_uint32, c.c_uint64, c.c_uint64, c.c_uint64, c.c_uint32,
                                           c.POINTER(c.c_uint64), c.c_uint32, c.POINTER(c.c_uint32)]
+        L.oracle_depth_counts.argtypes = [c.c_uint32, c.c_uint32, c.c_uint32, c.POINTER(U128), c.c_char_p,
+                                          c.c_size_t]
+        L.oracle_count_prefix_parallel.argtypes = [c.c_uint32, c.POINTER(c.c_uint32), c.c_uint32, c.c_uint32,
+                                                   c.POINTER(U128), c.c_char_p, c.c_size_t]
+        L.oracle_frontier_census.argtypes = [c.c_uint32, c.c_uint32, c.c_uint32, c.POINTER(c.c_uint64),
+                                             c.c_char_p, c.c_size_t]
         L.oracle_peak_rss_bytes.restype = c.c_uint64
         L.oracle_contract_rows_tensor.argtypes = [c.c_uint32, c.c_uint32, c.c_uint32, c.POINTER(TensorEntry),
                                                   c.c_uint32, c.c_uint32, LAYER_FN, c.c_void_p,
@@ -141,6 +147,17 @@ def count_prefix_dfs(n, cols):
     return out.value
 
 
+def count_prefix_parallel(n, cols, threads=8):
+    """count_prefix_dfs on `threads` threads (layer_census.cpp)."""
+    arr = (ctypes.c_uint32 * max(1, len(cols)))(*cols)
+    out = U128()
+    e = _err()
+    rc = lib().oracle_count_prefix_parallel(n, arr, len(cols), threads, ctypes.byref(out), e, 512)
+    if rc:
+        raise OracleError(e.value.decode())
+    return out.value
+
+
 def site_tensors():
     b = (ctypes.c_uint8 * 170)()
     c = (ctypes.c_uint8 * 153)()
@@ -184,3 +201,29 @@ def contract_rows_tensor(n, entries=None, symmetry=False, backend=SITEWISE, line
     if rc:
         raise OracleError(e.value.decode())
     return (out, dump) if layers else out
+
+
+def depth_counts(n, orbits=True, threads=8):
+    """[c_0 .. c_n]: c_r = the number of non-attacking r-row prefixes, each
+    weighted by the row-0 orbit weight when `orbits` (layer_census.cpp)."""
+    buf = (U128 * (n + 1))()
+    e = _err()
+    rc = lib().oracle_depth_counts(n, int(orbits), threads, buf, e, 512)
+    if rc:
+        raise OracleError(e.value.decode())
+    return [buf[i].value for i in range(n + 1)]
+
+
+CENSUS_KEYS = ("row", "input_states", "row_operator_matched", "completed_row_terms", "output_states")
+
+
+def frontier_census(n, orbits=True, threads=8):
+    """Per row: {row, input_states, row_operator_matched, completed_row_terms,
+    output_states} of the default row loop, by the grouped breadth-first pass
+    (layer_census.cpp)."""
+    buf = (ctypes.c_uint64 * (5 * n))()
+    e = _err()
+    rc = lib().oracle_frontier_census(n, int(orbits), threads, buf, e, 512)
+    if rc:
+        raise OracleError(e.value.decode())
+    return [dict(zip(CENSUS_KEYS, buf[5 * i: 5 * i + 5])) for i in range(n)]
