@@ -9,9 +9,14 @@ state-transitions per second, where transitions are the reference's
 time is the wall time per count with the device warmed.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 18]
-                       [--form plain|batched] [--no-cpu-baseline]
+                       [--form plain|batched] [--no-cpu-baseline] [--no-serving]
        python bench.py --form serving [--requests 2000] [--rates 50,200,800]
                        [--serve-n-min 10] [--serve-n-max 16]
+
+The plain form (the default) also streams a bounded mixed-N request set
+(300 requests, N uniform in [10, 18], 0..2 placed rows, saturating arrivals)
+through the request server after the timed region and adds its p50/p99
+latency, requests/s and the ratio to back-to-back plain counts as extra keys.
 
 The serving form streams requests (N, random non-attacking prefix of 0..2
 rows) with Poisson arrivals at each rate through the request server and
@@ -29,18 +34,19 @@ sys.path.insert(0, os.path.join(ROOT, "quantum-harness-peps-naive_amd"))
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, vendor figure
 
-# rocprofv3 kernel names of the engine's profiling labels (L: CRT lanes of
-# the record, 0 for packed 12-byte records; T: LDS table slots).
-ROCPROF_NAMES = {
-    "merge_count": "k_merge_expand<{L}, 0, {T}, 8, 256>",
-    "merge_scatter": "k_merge_expand<{L}, 1, {T}, 8, 512>",
-    "merge_last": "k_merge_expand<{L}, 2, {T}, 10, 256>",
-    "count_ub": "k_count_ub<{L}>",
-    "part_hist": "k_part_hist<{L}, 8, false, false>",
-    "part_hist_runs": "k_part_hist<{L}, 8, true, false>",
-    "part_hist_ub": "k_part_hist<{L}, 8, false, true>",
-    "part_scatter": "k_part_scatter<{L}, 8, false>",
-    "part_scatter_runs": "k_part_scatter<{L}, 8, true>",
+# rocprofv3 kernel names of the engine's profiling labels, as patterns over
+# every instantiation a label covers (L: CRT lanes of the record, 0 for
+# packed 12-byte records; the digit-width and table arguments vary by row).
+ROCPROF_PATTERNS = {
+    "merge_count": r"k_merge_expand<{L}, 0, \d+, \d+, \d+>",
+    "merge_scatter": r"k_merge_expand<{L}, 1, \d+, \d+, \d+>",
+    "merge_last": r"k_merge_expand<{L}, 2, \d+, \d+, \d+>",
+    "count_ub": r"k_count_ub<{L}>",
+    "part_hist": r"k_part_hist<{L}, \d+, false, false>",
+    "part_hist_runs": r"k_part_hist<{L}, \d+, true, false>",
+    "part_hist_ub": r"k_part_hist<{L}, \d+, false, true>",
+    "part_scatter": r"k_part_scatter<{L}, \d+, false>",
+    "part_scatter_runs": r"k_part_scatter<{L}, \d+, true>",
 }
 
 
@@ -49,19 +55,32 @@ def record_tag(result):
     return 0 if result.record_bytes == 12 else result.crt_lanes
 
 
-def measured_traffic(n, form, kernel, lanes, table_slots=2048):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3
-    FETCH_SIZE/WRITE_SIZE summary of the same workload (None if absent)."""
+def measured_traffic(n, form, kernel, lanes):
+    """HBM bytes per launch of the engine label `kernel` from the committed
+    rocprofv3 FETCH_SIZE/WRITE_SIZE summary of the same workload: the
+    call-weighted mean over every instantiation the label covers (None if
+    the summary or the label is absent)."""
+    import re
+
     path = os.path.join(ROOT, "profiles", f"kernels_n{n}_{form}.txt")
-    name = ROCPROF_NAMES.get(kernel, "").format(L=lanes, T=table_slots)
-    if not name or not os.path.exists(path):
+    pat = ROCPROF_PATTERNS.get(kernel)
+    if not pat or not os.path.exists(path):
         return None
+    rx = re.compile(pat.format(L=lanes) + "$")
+    calls, traffic = {}, {}
+    row = re.compile(r"^\s+(\S.*?)\s+(\d+)\s+([\d.]+)\s+([\d.]+)\s*$")
     for line in open(path):
         if line.startswith("traffic "):
             k, v = line[len("traffic "):].rstrip("\n").rsplit(" ", 1)
-            if k == name:
-                return float(v)
-    return None
+            traffic[k] = float(v)
+        elif not line.startswith("#"):
+            m = row.match(line)
+            if m and m.group(1) not in calls:
+                calls[m.group(1)] = int(m.group(2))
+    hit = [(calls[k], traffic[k]) for k in traffic if rx.match(k) and calls.get(k)]
+    if not hit:
+        return None
+    return sum(c * t for c, t in hit) / sum(c for c, _ in hit)
 
 
 def log(*a):
@@ -88,8 +107,11 @@ def host_threads():
 def cpu_baseline(n, budget_s):
     """The oracle's restatement of the reference's rayon path
     (contract_rows_d4_orbit_parallel_sort_reduce, src/lib.rs:5469-5626) run on
-    a bounded sample: whole 2-row prefix tasks of the same board, timed until
-    the budget is spent.  Rate = transitions (row_operator_matched) / s."""
+    a bounded sample: whole 2-row prefix tasks of the same board, stratified
+    over the first-row column (centre and edges first, then the columns
+    between), timed until the budget is spent.  Rate = transitions
+    (row_operator_matched) / s over the sampled tasks; the full count's time
+    is estimated as the plain count's transitions at that rate."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import _oracle
 
@@ -97,7 +119,26 @@ def cpu_baseline(n, budget_s):
     # the 2-row prefix tasks of the reference's task builder (row 0 left half)
     tasks = [(c0, c1) for c0 in range((n + 1) // 2) for c1 in range(n)
              if c1 != c0 and abs(c1 - c0) != 1]
-    order = [tasks[i] for i in range(0, len(tasks), max(1, len(tasks) // 8))]
+    half = (n + 1) // 2
+    # first-row columns in van der Corput order over [0, half): the sample
+    # spreads from the centre and the edge towards the columns in between
+    def vdc(i):
+        x, f = 0.0, 0.5
+        while i:
+            x += f * (i & 1)
+            i >>= 1
+            f /= 2
+        return x
+    c0_order = [half - 1, 0]
+    for i in range(4 * half):
+        c = min(half - 1, int(round(vdc(i) * (half - 1)))) if i else half - 1
+        if c not in c0_order:
+            c0_order.append(c)
+    c0_order += [c for c in range(half) if c not in c0_order]
+    order = []
+    for c0 in c0_order:
+        legal = [t for t in tasks if t[0] == c0]
+        order.append(legal[len(legal) // 2])
     done, matched, t_total = [], 0, 0.0
     for cand in order:
         t0 = time.perf_counter()
@@ -105,7 +146,8 @@ def cpu_baseline(n, budget_s):
         dt = time.perf_counter() - t0
         t_total += dt
         matched += r.row_operator_matched.value
-        done.append(list(cand))
+        done.append({"prefix": list(cand), "seconds": round(dt, 3),
+                     "transitions": r.row_operator_matched.value})
         log(f"cpu baseline task {cand}: {dt:.2f}s matched={r.row_operator_matched.value}")
         if t_total >= budget_s:
             break
@@ -114,10 +156,59 @@ def cpu_baseline(n, budget_s):
         "unit": "transitions/s",
         "cores": threads,
         "kind": "port",
-        "sample": (f"{len(done)} of {len(tasks)} two-row prefix tasks of N={n} "
-                   f"(prefixes {done}) through the oracle's restatement of the reference's "
-                   f"rayon sort-reduce (checked u128 records, {threads} threads, {host_cpu_model()})"),
+        "sample": (f"{len(done)} of {len(tasks)} two-row prefix tasks of N={n}, one per first-row column "
+                   f"in the order {[d['prefix'][0] for d in done]} (the median-index second-row column "
+                   f"of each), through the oracle's restatement of the reference's rayon sort-reduce "
+                   f"(checked u128 records, {threads} threads, {host_cpu_model()}); rate = their "
+                   f"transitions / their time"),
+        "tasks": done,
         "seconds": t_total,
+    }
+
+
+def serving_leg(pm, requests, n_min, n_max, seed, rate):
+    """A bounded mixed-N request stream through the request server after the
+    timed region: open-loop arrivals at `rate`, then the same requests as
+    back-to-back plain GPU counts.  Every served count must equal its
+    back-to-back count (and known_count for an empty prefix)."""
+    from peps_mi355x import serving as sv
+
+    reqs = sv.make_requests(seed, requests, rate, n_min, n_max)
+    with pm.Server() as server:
+        sv.run_open_loop(server, sv.make_requests(seed + 1, 40, 1e6, n_min, min(n_max, 14)))  # warm
+        st0 = server.stats()
+        outs = sv.run_open_loop(server, reqs, timeout_s=600,
+                                progress=lambda stage, d, t: log(f"serving: {stage} {d}/{t}"))
+        st1 = server.stats()
+    t0 = time.perf_counter()
+    alone = []
+    for i, (_, n, cols) in enumerate(reqs):
+        alone.append(pm.count_prefix(n, cols))
+        if (i + 1) % 100 == 0:
+            log(f"back-to-back: {i + 1}/{len(reqs)}")
+    b2b_s = time.perf_counter() - t0
+    bad = [(n, c) for (_, n, c), o, a in zip(reqs, outs, alone)
+           if o.status != 0 or o.count != a or (not c and a != pm.known_count(n))]
+    if bad:
+        raise SystemExit(f"serving: wrong or failed requests: {bad[:5]}")
+    summ = sv.summarize(outs)
+    b2b_rate = len(reqs) / b2b_s
+    return {
+        "requests": len(reqs),
+        "n_range": [n_min, n_max],
+        "prefix_rows": [0, 2],
+        "offered_rate": rate,
+        "seed": seed,
+        "requests_per_s": summ["requests_per_s"],
+        "p50_s": summ["p50_s"],
+        "p90_s": summ["p90_s"],
+        "p99_s": summ["p99_s"],
+        "max_s": summ["max_s"],
+        "percentile_rule": "sorted[(len - 1) * q] (src/bin/dfs_bitmask.rs:70-73)",
+        "launches": st1["launches"] - st0["launches"],
+        "back_to_back_requests_per_s": b2b_rate,
+        "vs_back_to_back": summ["requests_per_s"] / b2b_rate,
+        "checked": "every count equals the same request as a plain GPU count; empty prefixes equal known_count",
     }
 
 
@@ -204,7 +295,15 @@ def main():
     ap.add_argument("--autotune-seconds", type=float, default=240.0)
     ap.add_argument("--depth", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--cpu-seconds", type=float, default=30.0)
+    ap.add_argument("--profile-steps", type=int, default=1,
+                    help="untimed steps with per-kernel HIP-event timing after the timed region")
+    ap.add_argument("--no-serving", action="store_true",
+                    help="skip the mixed-N serving leg of the plain form")
+    ap.add_argument("--serve-requests", type=int, default=300)
+    ap.add_argument("--serve-leg-n-max", type=int, default=18)
+    ap.add_argument("--serve-rate", type=float, default=1e5,
+                    help="offered request rate of the serving leg (default: saturating)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -256,7 +355,7 @@ def main():
     if args.form == "plain":
         plain_matched = r.row_operator_matched.value if args.warmup else None
 
-    pm.profile_kernels(True)
+    # Timed steps run without kernel timing (no HIP events between launches).
     times, last = [], None
     t_all = time.perf_counter()
     for i in range(args.steps):
@@ -265,6 +364,12 @@ def main():
         times.append(time.perf_counter() - t0)
         log(f"step {i}: {times[-1]:.3f}s kernel={last.kernel_s:.3f}s")
     t_all = time.perf_counter() - t_all
+    # One more step with HIP-event timing of every launch, for the per-kernel
+    # figures and the roofline (outside the timed region).
+    pm.profile_kernels(True)
+    prof_steps = max(1, args.profile_steps)
+    for _ in range(prof_steps):
+        step()
     stats = pm.kernel_stats()
     pm.profile_kernels(False)
     if plain_matched is None:
@@ -280,15 +385,14 @@ def main():
             plain_matched = last.row_operator_matched.value
             transitions_source = f"{args.form} count (slices are not merged with each other)"
 
-
     ms_per_step = 1000.0 * t_all / args.steps
     value = plain_matched / (t_all / args.steps)
 
     per_kernel = {}
     for name, s in stats.items():
         gbs = s["algorithmic_bytes"] / s["seconds"] / 1e9 if s["seconds"] else 0.0
-        per_kernel[name] = {"launches_per_step": s["launches"] / args.steps,
-                            "ms_per_step": 1000.0 * s["seconds"] / args.steps,
+        per_kernel[name] = {"launches_per_step": s["launches"] / prof_steps,
+                            "ms_per_step": 1000.0 * s["seconds"] / prof_steps,
                             "avg_launch_us": 1e6 * s["seconds"] / max(1, s["launches"]),
                             "algorithmic_GBps": gbs}
     dom = max(stats.items(), key=lambda kv: kv[1]["seconds"]) if stats else None
@@ -301,6 +405,12 @@ def main():
                     "traffic": measured_traffic(n, args.form, name, record_tag(last)),
                     "avg_launch_us": round(1e6 * s["seconds"] / s["launches"], 3),
                     "algorithmic_bytes_per_launch": s["algorithmic_bytes"] / s["launches"]}
+
+    serving = None
+    if args.form == "plain" and not args.no_serving:
+        serving = serving_leg(pm, args.serve_requests, args.serve_n_min, args.serve_leg_n_max, args.seed,
+                              args.serve_rate)
+        log(f"serving leg: {json.dumps(serving)}")
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -341,6 +451,11 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
+    if serving:
+        # mixed-N serving after the timed region (extra keys; `value` is the count rate above)
+        out.update({"p50_s": serving["p50_s"], "p99_s": serving["p99_s"],
+                    "requests_per_s": serving["requests_per_s"],
+                    "vs_back_to_back": serving["vs_back_to_back"], "serving": serving})
     if cpu and cpu.get("value"):
         out["cpu_time_to_count_est_s"] = plain_matched / cpu["value"]
     print(json.dumps(out), flush=True)
