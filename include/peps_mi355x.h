@@ -160,6 +160,25 @@ int peps_contract_rows_with_modes(uint32_t n, uint32_t symmetry, const peps_opti
 int peps_count_prefix(uint32_t n, const uint32_t* cols, uint32_t k, const peps_options* opts,
                       peps_u128* count, char* err, size_t err_cap);
 
+/* The same count without allocating.  No counterpart in the reference: for
+ * callers that cannot allocate device memory (code that prepares HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Every
+ * device buffer is a fixed slice of one static device array of 256 MiB per
+ * GPU (PEPS_NO_ALLOC_SCRATCH_BYTES), part of the library's code object and
+ * held from the library's load on that GPU until the process ends; kernels
+ * run on the device's null stream and nothing is allocated on the device or
+ * as pinned host memory (host bookkeeping uses the C++ heap).  A board whose
+ * rows need more than the scratch holds (N = 14 fits; N = 15 and up only as
+ * prefixes with small frontiers) fails with PEPS_E_MEMORY.  Calls on one GPU
+ * share its array, which keeps the last call's contents: each call first
+ * clears its counters with a kernel, and the library runs such calls one at
+ * a time per GPU (a lock), so calls that would overlap wait.  graph_mode is
+ * rejected; merge_waves is fixed at 1.  Same arguments, results and errors
+ * as peps_count_prefix otherwise. */
+#define PEPS_NO_ALLOC_SCRATCH_BYTES (256ull << 20)
+int peps_count_prefix_no_alloc(uint32_t n, const uint32_t* cols, uint32_t k, const peps_options* opts,
+                               peps_u128* count, char* err, size_t err_cap);
+
 /* Batched form: all prefix tasks of depth `split_depth` (reference
  * build_wide_crt_tasks semantics: orbit weight on row 0) contracted as
  * independent slices (slice id above the 3N-bit key) in groups of

@@ -113,6 +113,53 @@ int peps_count_prefix(uint32_t n, const uint32_t* cols, uint32_t k, const peps_o
   return PEPS_OK;
 }
 
+int peps_count_prefix_no_alloc(uint32_t n, const uint32_t* cols, uint32_t k, const peps_options* opts,
+                               peps_u128* count, char* err, size_t err_cap) {
+  if (!count) return fail(PEPS_E_INVALID, "null count pointer", err, err_cap);
+  *count = abi(0);
+  std::string e;
+  if (n > 42) return fail(PEPS_E_INVALID, "the packed u128 virtual-boundary backend supports N <= 42", err, err_cap);
+  if (k > n) return fail(PEPS_E_INVALID, "prefix longer than the board", err, err_cap);
+  if (k && !cols) return fail(PEPS_E_INVALID, "null prefix", err, err_cap);
+  for (uint32_t i = 0; i < k; ++i)
+    if (cols[i] >= n) return fail(PEPS_E_INVALID, "prefix column outside the board", err, err_cap);
+  if (n == 0) {  // src/lib.rs:5295-5307
+    *count = abi(1);
+    return PEPS_OK;
+  }
+  int rc = check_board(n, &e);
+  if (rc) return fail(rc, e, err, err_cap);
+  uint64_t key = 0;
+  if (k && !peps::prefix_state(n, cols, k, &key)) return PEPS_OK;  // attacking prefix: C gives 0
+  if (k == n) {
+    *count = abi(1);
+    return PEPS_OK;
+  }
+  Resolved r;
+  rc = peps::resolve_options(n, opts, &r, &e);
+  if (rc) return fail(rc, e, err, err_cap);
+  if (r.graph) return fail(PEPS_E_INVALID, "graph_mode is not available without allocation", err, err_cap);
+  peps::RunSpec spec;
+  spec.n = n;
+  spec.first_row = k;
+  spec.orbit_first_row = k == 0;  // an empty prefix is contract_rows
+  spec.seeds.push_back(peps::Seed{key, 1});
+  peps::apply_resolved(r, &spec);
+  spec.use_graph = false;
+  spec.merge_waves = 1;   // the scratch's histogram slices hold one wave of workgroups
+  spec.budget_bytes = 0;  // the scratch is the budget
+  peps::Engine* eng = peps::Engine::get_scratch((int)r.device, &e);
+  if (!eng) return fail(PEPS_E_DEVICE, e, err, err_cap);
+  peps::RunOut ro;
+  {
+    std::lock_guard<std::mutex> lk(eng->mutex());
+    rc = eng->run(spec, &ro, &e);
+  }
+  if (rc) return fail(rc, e, err, err_cap);
+  *count = abi(ro.count);
+  return PEPS_OK;
+}
+
 int peps_count_batched(uint32_t n, uint32_t split_depth, const peps_options* opts,
                        peps_contraction_result* out, uint32_t* tasks_out, peps_u128* task_counts,
                        uint64_t task_cap, char* err, size_t err_cap) {

@@ -130,9 +130,19 @@ struct KernelStat {
   double bytes = 0;
 };
 
+// Device bytes of the no-allocation engine's scratch array (one per device).
+constexpr size_t kScratchBytes = (size_t)256 << 20;
+
 class Engine {
  public:
   static Engine* get(int device, std::string* err);  // one engine per device
+  // The no-allocation engine of `device`: every device buffer is a fixed
+  // slice of the library's static scratch array, kernels run on the device's
+  // null stream, and nothing is allocated on the device or as pinned host
+  // memory, at set-up or later.  A row that does not fit its slice fails
+  // with PEPS_E_MEMORY.  Fused pipeline only (no graphs, no slices, no
+  // materialised layers, no kernel timing).
+  static Engine* get_scratch(int device, std::string* err);
   // A run in kRecordsAuto format whose weights outgrow the packed records is
   // repeated with lane records (same result, larger records).
   int run(const RunSpec& spec, RunOut* out, std::string* err);
@@ -154,6 +164,9 @@ class Engine {
  private:
   explicit Engine(int device) : device_(device) {}
   int init(std::string* err);
+  int init_scratch(std::string* err);
+  bool fixed_ = false;                  // buffers are fixed scratch slices
+  uint64_t h_local_[kNumCounters] = {}; // host counter mirror of the scratch engine
   int ensure_region(int idx, size_t bytes, std::string* err);
   template <typename T>
   int ensure_buffer(T** ptr, size_t* cap, size_t count, std::string* err);

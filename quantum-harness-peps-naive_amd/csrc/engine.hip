@@ -40,8 +40,111 @@ std::map<int, std::unique_ptr<Engine>>& engines() {
   static std::map<int, std::unique_ptr<Engine>> m;
   return m;
 }
+std::map<int, std::unique_ptr<Engine>>& scratch_engines() {
+  static std::map<int, std::unique_ptr<Engine>> m;
+  return m;
+}
 
 }  // namespace
+
+// The no-allocation engine's device memory: one array of kScratchBytes per
+// device, part of the library's code object, held from the library's load
+// on that device until the process ends.  Host code never passes the
+// array's name to a kernel or a copy (on the host it is a stand-in symbol,
+// not a device address): init_scratch looks up each device's copy with
+// hipGetSymbolAddress, with that device current, and the engine uses that
+// address only.
+static __device__ __align__(256) uint8_t scratch[kScratchBytes];
+
+Engine* Engine::get_scratch(int device, std::string* err) {
+  std::lock_guard<std::mutex> lk(g_engines_mu);
+  auto& m = scratch_engines();
+  auto it = m.find(device);
+  if (it != m.end()) return it->second.get();
+  std::unique_ptr<Engine> e(new Engine(device));
+  if (e->init_scratch(err) != 0) return nullptr;
+  Engine* raw = e.get();
+  m[device] = std::move(e);
+  return raw;
+}
+
+int Engine::init_scratch(std::string* err) {
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count == 0) {
+    *err = "no HIP device available (the MI355X path needs a gfx950 GPU)";
+    return 3;
+  }
+  if (device_ < 0 || device_ >= count) {
+    *err = "HIP device ordinal out of range";
+    return 1;
+  }
+  HIP_TRY(hipSetDevice(device_));
+  hipDeviceProp_t prop;
+  HIP_TRY(hipGetDeviceProperties(&prop, device_));
+  if (std::string(prop.gcnArchName).find("gfx950") == std::string::npos) {
+    *err = std::string("device is ") + prop.gcnArchName + ", this library is built for gfx950";
+    return 3;
+  }
+  cu_count_ = (uint32_t)std::max(1, prop.multiProcessorCount);
+  void* base = nullptr;
+  HIP_TRY(hipGetSymbolAddress(&base, HIP_SYMBOL(scratch)));
+  if (!base) {
+    *err = "scratch array has no device address";
+    return 3;
+  }
+  fixed_ = true;
+  stream_ = nullptr;  // the device's null stream: nothing to create
+  h_counters_ = h_local_;
+  // Fixed slices, 256-byte aligned.  The small buffers are sized for a
+  // merge grid of one wave of resident workgroups (the no-allocation runs
+  // use merge_waves = 1): COUNT histograms of 2^10 digits x 1024
+  // workgroups; the two candidate regions share what is left.
+  uint8_t* p = (uint8_t*)base;
+  size_t left = kScratchBytes;
+  auto take = [&](size_t bytes) -> void* {
+    bytes = (bytes + 255) & ~(size_t)255;
+    void* r = p;
+    p += bytes;
+    left -= bytes;
+    return r;
+  };
+  constexpr size_t kCells = (size_t)1 << 20;
+  counters_ = (uint64_t*)take(sizeof(uint64_t) * kNumCounters);
+  counters_rows_cap_ = 64 * kNumCounters;
+  counters_rows_ = (uint64_t*)take(8 * counters_rows_cap_);
+  seed_cap_ = 8192;
+  seed_ = (uint64_t*)take(8 * seed_cap_);
+  slice_cap_ = 1024;
+  slice_sums_ = (uint64_t*)take(8 * slice_cap_);
+  hist_cap_ = kCells;
+  hist_ = (uint32_t*)take(4 * hist_cap_);
+  offsets_cap_ = kCells;
+  offsets_ = (uint64_t*)take(8 * offsets_cap_);
+  partial_cap_ = kCells / 4096 + 64;
+  partial_ = (uint64_t*)take(8 * partial_cap_);
+  for (int i = 0; i < 2; ++i) {
+    segs_cap_[i] = (size_t)1 << 17;
+    segs_[i] = (uint64_t*)take(8 * segs_cap_[i]);
+  }
+  run_start_cap_ = kCells;
+  run_start_ = (uint64_t*)take(8 * run_start_cap_);
+  run_len_cap_ = kCells;
+  run_len_ = (uint32_t*)take(4 * run_len_cap_);
+  ubh_cap_ = kCells / 2 + 64;
+  ubh_ = (uint64_t*)take(8 * ubh_cap_);
+  const size_t half = (left / 2) & ~(size_t)255;
+  for (int i = 0; i < 2; ++i) {
+    region_bytes_[i] = half;
+    region_[i] = take(half);
+  }
+  budget_ = arena_bytes();
+  std::string cerr;
+  if (!compile_row_operator(site_tensor_c(site_tensor_b()), &occupied_, &cerr)) {
+    *err = cerr;
+    return 1;
+  }
+  return 0;
+}
 
 Engine* Engine::get(int device, std::string* err) {
   std::lock_guard<std::mutex> lk(g_engines_mu);
@@ -92,6 +195,11 @@ uint64_t Engine::arena_bytes() const {
 
 int Engine::ensure_region(int idx, size_t bytes, std::string* err) {
   if (region_bytes_[idx] >= bytes) return 0;
+  if (fixed_) {
+    *err = "the no-allocation scratch cannot hold this board: a row needs " + std::to_string(bytes) +
+           " bytes per candidate buffer, the scratch has " + std::to_string(region_bytes_[idx]);
+    return 4;
+  }
   const auto ta = Clock::now();
   struct Tally {
     double* acc;
@@ -151,6 +259,11 @@ uint64_t Engine::candidate_capacity(int lanes, int records) const {
 template <typename T>
 int Engine::ensure_buffer(T** ptr, size_t* cap, size_t count, std::string* err) {
   if (*cap >= count) return 0;
+  if (fixed_) {
+    *err = "the no-allocation scratch cannot hold this board: a launch needs " + std::to_string(count) +
+           " entries of a buffer whose scratch slice holds " + std::to_string(*cap);
+    return 4;
+  }
   struct Tally {
     double* acc;
     Clock::time_point t;
@@ -350,7 +463,7 @@ int Engine::prepare(const RunSpec& spec, std::string* err) {
     *err = "engine: nothing to contract";
     return 1;
   }
-  if (budget_ == 0 || (spec.budget_bytes && spec.budget_bytes != budget_)) {
+  if (!fixed_ && (budget_ == 0 || (spec.budget_bytes && spec.budget_bytes != budget_))) {
     if (spec.budget_bytes) {
       budget_ = spec.budget_bytes;
     } else {
@@ -507,6 +620,11 @@ int Engine::run_many_once(const std::string& key, const std::vector<RunSpec>& sp
 }
 
 int Engine::run(const RunSpec& spec, RunOut* out, std::string* err) {
+  if (fixed_ && (spec.use_graph || spec.materialize || spec.site || spec.per_slice || spec.slice_bits)) {
+    *err = "the no-allocation engine runs the fused row pipeline of one prefix only";
+    return 1;
+  }
+  if (fixed_ && profiling_) set_profiling(false);
   const int rc = run_once(spec, out, err);
   if (rc != kRcPackedOverflow) return rc;
   if (spec.records == kRecordsPacked) {
@@ -542,7 +660,7 @@ int Engine::run_once(const RunSpec& spec, RunOut* out, std::string* err) {
     *err = "engine: nothing to contract";
     return 1;
   }
-  if (budget_ == 0 || (spec.budget_bytes && spec.budget_bytes != budget_)) {
+  if (!fixed_ && (budget_ == 0 || (spec.budget_bytes && spec.budget_bytes != budget_))) {
     if (spec.budget_bytes) {
       budget_ = spec.budget_bytes;
     } else {
@@ -753,8 +871,9 @@ int Engine::pipeline(const RunSpec& spec, Plan* plan, bool eager, RunOut* out, s
     out->device_peak_bytes = 0;
     out->peak_write_bound = 0;
     out->residues.assign(L, 0);
-    HIP_TRY(hipEventRecord(ev_start_, stream_));
+    if (ev_start_) HIP_TRY(hipEventRecord(ev_start_, stream_));
   }
+  const auto tk0 = Clock::now();
   LAUNCH("fill", 8.0 * ncount, launch_fill(counters_rows_, ncount, 0, stream_));
   if (nslice) LAUNCH("fill", 8.0 * nslice, launch_fill(ssum, nslice, 0, stream_));
 
@@ -1082,11 +1201,17 @@ int Engine::pipeline(const RunSpec& spec, Plan* plan, bool eager, RunOut* out, s
   }
   if (!eager) return 0;
 
-  HIP_TRY(hipEventRecord(ev_stop_, stream_));
-  HIP_TRY(hipEventSynchronize(ev_stop_));
-  float ms = 0;
-  HIP_TRY(hipEventElapsedTime(&ms, ev_start_, ev_stop_));
-  out->kernel_s = ms / 1000.0;
+  if (ev_start_) {
+    HIP_TRY(hipEventRecord(ev_stop_, stream_));
+    HIP_TRY(hipEventSynchronize(ev_stop_));
+    float ms = 0;
+    HIP_TRY(hipEventElapsedTime(&ms, ev_start_, ev_stop_));
+    out->kernel_s = ms / 1000.0;
+  } else {
+    // no events (the no-allocation engine): host time of the row pipeline
+    HIP_TRY(hipStreamSynchronize(stream_));
+    out->kernel_s = std::chrono::duration<double>(Clock::now() - tk0).count();
+  }
   // The partition and expansion kernels flag capacity violations in the
   // spare counter row n.
   std::vector<uint64_t> spare(kNumCounters);

@@ -59,6 +59,9 @@ int resolve_options(unsigned n, const peps_options* o, Resolved* r, std::string*
 // N > 42: the reference's text (src/lib.rs:5293); 22..42: the u64 key limit.
 int check_board(unsigned n, std::string* err);
 
+// Copies the resolved lanes and tuning into `spec`.
+void apply_resolved(const Resolved& r, RunSpec* spec);
+
 // Runs `spec` on the resolved device with the resolved tuning.
 int run_engine(const Resolved& r, RunSpec& spec, RunOut* ro, std::string* err);
 

@@ -20,6 +20,7 @@ from .api import (  # noqa: F401
     contract_rows_with_tensor,
     count_batched,
     count_prefix,
+    count_prefix_no_alloc,
     crt_plan,
     crt_reconstruct,
     device_info,

@@ -103,6 +103,8 @@ def load():
                                                 c.c_char_p, c.c_size_t]
     L.peps_count_prefix.argtypes = [c.c_uint32, P(c.c_uint32), c.c_uint32, P(Options), P(U128),
                                     c.c_char_p, c.c_size_t]
+    L.peps_count_prefix_no_alloc.argtypes = [c.c_uint32, P(c.c_uint32), c.c_uint32, P(Options), P(U128),
+                                             c.c_char_p, c.c_size_t]
     L.peps_count_batched.argtypes = [c.c_uint32, c.c_uint32, P(Options), P(ContractionResult),
                                      P(c.c_uint32), P(U128), c.c_uint64, c.c_char_p, c.c_size_t]
     L.peps_known_count.argtypes = [c.c_uint32, P(U128)]
@@ -185,6 +187,19 @@ def count_prefix(n: int, cols, options: PepsOptions = None) -> int:
     e = _err()
     ref, _keep = _opts(options)
     _check(load().peps_count_prefix(n, arr, len(cols), ref, ctypes.byref(out), e, len(e)), e)
+    return out.value
+
+
+def count_prefix_no_alloc(n: int, cols, options: PepsOptions = None) -> int:
+    """count_prefix through peps_count_prefix_no_alloc: no device allocation,
+    every buffer in the library's 256 MiB static scratch (PEPS_E_MEMORY for
+    boards whose rows do not fit it)."""
+    cols = list(cols)
+    arr = (ctypes.c_uint32 * max(1, len(cols)))(*cols)
+    out = U128()
+    e = _err()
+    ref, _keep = _opts(options)
+    _check(load().peps_count_prefix_no_alloc(n, arr, len(cols), ref, ctypes.byref(out), e, len(e)), e)
     return out.value
 
 
