@@ -131,7 +131,9 @@ typedef struct {
                                 1 every C entry per partial-row visit (benchmarks/n14_layers.csv) */
   uint32_t record_format;    /* low byte, PEPS_RECORDS_*: candidate record format of the fused
                                 row pipeline; bits 8..15, if nonzero, narrow the packed weight
-                                field to that many bits (exercises the overflow path) */
+                                field to that many bits (exercises the overflow path); bits
+                                16..19, PEPS_FORCE_* << 16, make the pipeline take its fallback
+                                paths (tests; results never change) */
 } peps_options;
 
 /* Candidate records of the fused row pipeline.  AUTO: packed 12-byte records
@@ -141,6 +143,18 @@ typedef struct {
  * bytes for 1-2 primes, 24 for 3-4).  PACKED: packed only; a weight overflow
  * fails with PEPS_E_OVERFLOW.  Counts are identical in every format. */
 enum { PEPS_RECORDS_AUTO = 0, PEPS_RECORDS_LANES = 1, PEPS_RECORDS_PACKED = 2 };
+
+/* Fallback paths of the fused row pipeline (record_format bits 16..19).
+ * The partition writes fixed-capacity buckets by claims and the SCATTER
+ * writes capped runs sized from fanout sums; these bits force the exact
+ * histogram levels, upper-bound-count runs, or capacities too small to hold
+ * (so every such step overflows and is repeated by its fallback). */
+enum {
+  PEPS_FORCE_EXACT_LEVELS = 1,
+  PEPS_FORCE_UPPER_RUNS = 2,
+  PEPS_FORCE_LEVEL_OVERFLOW = 4,
+  PEPS_FORCE_CAP_OVERFLOW = 8
+};
 
 /* ---------------------------------------------------------------- solve -- */
 
@@ -318,7 +332,7 @@ int peps_sha256(const void* data, size_t len, uint8_t out[32]);
  * them as the slices of engine launches.  Each launch takes tasks of one
  * (N, D) class from every active request of that class (continuous batching:
  * a request that arrives while a launch runs joins the next launch of its
- * class), admitted while the projected candidate bytes fit 80% of one HBM
+ * class), admitted while the projected candidate bytes fit 55% of one HBM
  * candidate buffer.  Frontiers stay in HBM; a launch over budget is split.
  * The count of a request is the sum of its tasks' slice counts, so it is the
  * same whether it ran alone or in a mix.  There is no reference counterpart:

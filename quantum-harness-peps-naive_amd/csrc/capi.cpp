@@ -259,9 +259,9 @@ int peps_count_batched(uint32_t n, uint32_t split_depth, const peps_options* opt
     absorb(g0, g1, ro);
   };
   while (next < tasks.size()) {
-    // Admit tasks while their projected candidates fit 80% of a buffer.
+    // Admit tasks while their projected candidates fit kAdmitFraction of a buffer.
     const double ratio = proj.ratio(n, split_depth);
-    const double cap = 0.8 * (double)eng->candidate_capacity((int)r.primes.size(), r.records);
+    const double cap = peps::kAdmitFraction * (double)eng->candidate_capacity((int)r.primes.size(), r.records);
     const size_t g0 = next;
     size_t g1 = next + 1;
     double load = proxy[g0] * ratio;
@@ -276,7 +276,7 @@ int peps_count_batched(uint32_t n, uint32_t split_depth, const peps_options* opt
     auto exceeded = [&](size_t a, size_t b) {
       double psum = 0;
       for (size_t i = a; i < b; ++i) psum += proxy[i];
-      proj.exceeded(n, split_depth, psum, cap / 0.8);
+      proj.exceeded(n, split_depth, psum, cap / peps::kAdmitFraction);
     };
     rc = peps::run_task_range(r, n, split_depth, tasks, g0, g1, record, &e, exceeded);
     if (rc) return fail(rc, e, err, err_cap);

@@ -35,6 +35,18 @@ enum RecordFormat : int {
   kRecordsPacked = 2,  // packed only: a weight overflow fails the run
 };
 
+// Fallback paths a run can be made to take (the results never change):
+// exact histogram partition levels, SCATTER runs sized by the upper-bound
+// count, fixed-capacity buckets too small (every such level overflows and is
+// repeated exactly), caps too small (capped runs overflow and the SCATTER is
+// repeated with upper-bound offsets).
+enum ForcePath : uint32_t {
+  kForceExactLevels = 1,
+  kForceUpperRuns = 2,
+  kForceLevelOverflow = 4,
+  kForceCapOverflow = 8,
+};
+
 struct RunSpec {
   unsigned n = 0;
   unsigned first_row = 0;          // contract rows [first_row, n)
@@ -58,6 +70,7 @@ struct RunSpec {
   bool bound_check = true;         // per-row sums and the count must not exceed N!
   int records = kRecordsAuto;      // fused pipeline record format (materialised: lanes)
   uint32_t packed_weight_bits = kPackedMaxWeightBits;  // narrower packed weights (tests)
+  uint32_t force = 0;              // kForce* bits: take the fallback paths (tests)
   // Materialised mode: called after every row with the merged layer
   // (keys[S], residues[l * S + i]); a nonzero return aborts the run.
   std::function<int(unsigned row, const std::vector<uint64_t>& keys, const std::vector<uint32_t>& res,
@@ -101,12 +114,18 @@ struct RunOut {
 // repeat of the same spec has the same sizes and its whole row pipeline can be
 // captured once into a HIP graph and replayed without host synchronisation.
 struct RowPlan {
+  // How the SCATTER finds its write positions: exact (COUNT merge + scan),
+  // capped runs from the fanout sums, or runs sized by the upper-bound
+  // count of every record's successors.  Capped and upper-bound runs leave
+  // gaps, which the next row's first partition level reads in run mode.
+  enum Mode : int { kExact = 0, kCapped = 1, kUpper = 2 };
+  int mode = kExact;
   uint64_t C_next = 0;             // candidates of the next row (0: none)
-  uint64_t C_count = 0;            // what the COUNT pass reported (C_next, or its upper bound)
-  bool upper_bound = false;        // COUNT without merging; SCATTER leaves gaps, level 1 reads runs
+  uint64_t C_count = 0;            // records the SCATTER's layout spans (C_next, or more with gaps)
   uint32_t W = 0;                  // merge-expand workgroups for this row
   uint32_t d1 = 0;                 // next row's level-1 digit width
   std::vector<unsigned> pdigits;   // next row's partition digit widths
+  std::vector<uint64_t> pcap;      // their bucket capacities (0: exact histogram level)
 };
 
 struct Plan {
@@ -114,11 +133,12 @@ struct Plan {
   u128 matched_first = 0;
   uint32_t d1_first = 0;
   std::vector<unsigned> pdigits_first;
+  std::vector<uint64_t> pcap_first;
   uint64_t tiles = 1, per_tile = 1;
   std::vector<RowPlan> rows;       // rows [first_row, n)
   size_t region_need = 0, hist_need = 0, offsets_need = 0, partial_need = 0;
   size_t segs_need[2] = {0, 0};
-  size_t run_start_need = 0, run_len_need = 0, ubh_need = 0;
+  size_t run_start_need = 0, run_len_need = 0, fan_need = 0;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   uint64_t generation = 0;         // buffer generation the graph was captured against
@@ -220,8 +240,8 @@ class Engine {
   size_t run_start_cap_ = 0;
   uint32_t* run_len_ = nullptr;        // upper-bound rows: SCATTER run lengths [D1][W]
   size_t run_len_cap_ = 0;
-  uint64_t* ubh_ = nullptr;            // upper-bound COUNT of the last partition level, u32 [D][W]
-  size_t ubh_cap_ = 0;
+  uint64_t* fan_ = nullptr;            // fanout sums U[W], caps [W], their prefix sums [W + 1]
+  size_t fan_cap_ = 0;
   uint64_t* counters_rows_ = nullptr;  // per-row counters, (n + 1) * kNumCounters
   size_t counters_rows_cap_ = 0;
   uint64_t* seed_ = nullptr;           // seed frontier (SoA, seed_image layout)

@@ -7,6 +7,7 @@
 #include "engine.h"
 
 #include <algorithm>
+#include <cmath>
 #include <chrono>
 #include <cstring>
 #include <map>
@@ -130,8 +131,8 @@ int Engine::init_scratch(std::string* err) {
   run_start_ = (uint64_t*)take(8 * run_start_cap_);
   run_len_cap_ = kCells;
   run_len_ = (uint32_t*)take(4 * run_len_cap_);
-  ubh_cap_ = kCells / 2 + 64;
-  ubh_ = (uint64_t*)take(8 * ubh_cap_);
+  fan_cap_ = kCells / 2 + 64;
+  fan_ = (uint64_t*)take(8 * fan_cap_);
   const size_t half = (left / 2) & ~(size_t)255;
   for (int i = 0; i < 2; ++i) {
     region_bytes_[i] = half;
@@ -379,7 +380,9 @@ std::string error_flag_text(uint64_t bits, const std::string& where) {
   if (bits & 1) add("LDS merge table overflow");
   if (bits & 2) add("a kernel write reached past its buffer's planned capacity");
   if (bits & 8) add("sitewise partial-row limit exceeded");
-  if (bits & ~11ull) add("unknown device error flag");
+  if (bits & 16) add("a capped SCATTER run overflowed");
+  if (bits & 32) add("a fixed-capacity bucket overflowed");
+  if (bits & ~59ull) add("unknown device error flag");
   return s + where;
 }
 
@@ -438,10 +441,10 @@ std::string plan_key(const RunSpec& s) {
     }
   }
   char buf[256];
-  snprintf(buf, sizeof(buf), "%u/%u/%d/%u/%zu/%u/%u/%u/%u/%u/%d/%zu/%llx/%d/%u", s.n, s.first_row,
+  snprintf(buf, sizeof(buf), "%u/%u/%d/%u/%zu/%u/%u/%u/%u/%u/%d/%zu/%llx/%d/%u/%u", s.n, s.first_row,
            (int)s.orbit_first_row, s.slice_bits, s.primes.size(), s.tile_states, s.radix_bits,
            s.expand_bits, s.merge_passes, s.merge_waves, (int)s.per_slice, s.seeds.size(),
-           (unsigned long long)h, s.records, s.packed_weight_bits);
+           (unsigned long long)h, s.records, s.packed_weight_bits, s.force);
   return buf;
 }
 
@@ -529,7 +532,7 @@ int Engine::run_many_once(const std::string& key, const std::vector<RunSpec>& sp
     if (!rc) rc = ensure_buffer(&segs_[1], &segs_cap_[1], plan.segs_need[1], err);
     if (!rc) rc = ensure_buffer(&run_start_, &run_start_cap_, plan.run_start_need, err);
     if (!rc) rc = ensure_buffer(&run_len_, &run_len_cap_, plan.run_len_need, err);
-    if (!rc) rc = ensure_buffer(&ubh_, &ubh_cap_, plan.ubh_need, err);
+    if (!rc) rc = ensure_buffer(&fan_, &fan_cap_, plan.fan_need, err);
   }
   if (rc) return rc;
   // Per-group seeds, counters and slice sums, so that every group's pipeline
@@ -695,7 +698,7 @@ int Engine::run_once(const RunSpec& spec, RunOut* out, std::string* err) {
       if (!rc) rc = ensure_buffer(&segs_[1], &segs_cap_[1], plan.segs_need[1], err);
       if (!rc) rc = ensure_buffer(&run_start_, &run_start_cap_, plan.run_start_need, err);
       if (!rc) rc = ensure_buffer(&run_len_, &run_len_cap_, plan.run_len_need, err);
-      if (!rc) rc = ensure_buffer(&ubh_, &ubh_cap_, plan.ubh_need, err);
+      if (!rc) rc = ensure_buffer(&fan_, &fan_cap_, plan.fan_need, err);
       if (!rc) rc = ensure_buffer(&counters_rows_, &counters_rows_cap_, (size_t)(n + 1) * kNumCounters, err);
       if (!rc) rc = ensure_buffer(&seed_, &seed_cap_, seed_host.size(), err);
       if (!rc && spec.per_slice)
@@ -904,11 +907,12 @@ int Engine::pipeline(const RunSpec& spec, Plan* plan, bool eager, RunOut* out, s
   }
 
   int cur = 0;  // buffer holding the current row's final buckets
-  uint64_t nseg = 1;
   int sidx = 0;
+  Buckets bk{nullptr, 0, 1};  // the current row's final buckets (segs_[sidx])
   std::vector<uint64_t> final_res(L, 0);
   uint64_t S_in = S0;
   unsigned levels = 0;
+  uint64_t* const spare = row_counters(0) + n * kNumCounters;
 
   // Geometry of the merge rows (no orbit weights) and the COUNT digit width.
   RowGeom gm = g;
@@ -916,76 +920,116 @@ int Engine::pipeline(const RunSpec& spec, Plan* plan, bool eager, RunOut* out, s
   gm.orbit_mask = gm.board_mask;
   gm.centre_bit = 0;
   const uint32_t CB = std::min<uint32_t>(kFineBits, std::max(1u, spec.expand_bits));
-  // Set when the last partition level of the current row also did its
-  // upper-bound COUNT into ubh_ (for a merge launch of ub_W workgroups).
-  bool ub_fused = false;
-  uint32_t ub_W = 0;
+  // Set when the last partition level of the current row summed its
+  // records' fanout into fan_ (for a merge launch of fan_W workgroups).
+  bool fan_ready = false;
+  uint32_t fan_W = 0;
 
-  // Partition levels after the level-1 digit d1.  With `runs`, the first
-  // level reads the SCATTER's runs (upper-bound offsets, gaps between runs)
-  // and its scan writes the exact candidate count into `exact`.  With
-  // `count_ub` (the row's merge will take the upper-bound COUNT), the last
-  // level's histogram launch also counts the successors of the buckets it
-  // defines, when a segment's buckets span few enough merge workgroups.
-  auto partition_to_final = [&](int from, unsigned d1, const std::vector<unsigned>& pd, uint64_t Crow,
-                                const Runs* runs, uint64_t* exact, bool count_ub) -> int {
+  // Reads one counter block back (eager runs only).
+  auto read_block = [&](const uint64_t* dev) -> int {
+    HIP_TRY(hipMemcpyAsync(h_counters_, dev, sizeof(uint64_t) * kNumCounters, hipMemcpyDeviceToHost, stream_));
+    SYNC_AND_RESOLVE();
+    return 0;
+  };
+
+  // Partition levels pd after the level-1 digit d1 of the buckets `in` in
+  // region `from` (or, with `runs`, of the SCATTER's runs there).  A level
+  // writes fixed-capacity buckets by per-stage claims when the region leaves
+  // every bucket a wide margin over its mean fill; otherwise, or when a
+  // claim overflowed (eager runs repeat the level), it takes the exact
+  // histogram + scan path.  pfix/pcap record each level's capacity (0:
+  // exact).  With `fan`, the last level also sums the records' fanout for
+  // the merge of these buckets into fan_, and their total into
+  // rowc[kFanout].
+  auto partition_to_final = [&](int from, Buckets in, unsigned d1, const std::vector<unsigned>& pd,
+                                std::vector<uint64_t>* pcap, uint64_t Crow, const Runs* runs, uint64_t* rowc,
+                                bool fan) -> int {
     cur = from;
-    nseg = (uint64_t)1 << d1;
+    bk = in;
     levels = d1 ? 1 : 0;
     unsigned used = d1;
     const Runs* rn = runs;  // the first level only
-    ub_fused = false;
+    fan_ready = false;
+    if (eager) pcap->assign(pd.size(), 0);
     for (size_t li = 0; li < pd.size(); ++li) {
       const unsigned dk = pd[li];
       used += dk;
       const int to = cur ^ 1;
+      const uint64_t nseg = bk.nseg;
+      const uint64_t nout = nseg << dk;
+      const bool last_level = li + 1 == pd.size();
       uint32_t chunks = partition_chunks(Crow, nseg);
       if (rn) chunks = std::max<uint32_t>(chunks, (rn->per_seg + kMaxChunkRuns - 1) / kMaxChunkRuns);
-      const size_t cells = (size_t)(nseg << dk) * chunks;
-      int e = grow(to, Crow * rb);
-      if (!e) e = need(&segs_[sidx ^ 1], &segs_cap_[sidx ^ 1], &plan->segs_need[sidx ^ 1], (nseg << dk) + 1);
-      if (!e) e = need(&hist_, &hist_cap_, &plan->hist_need, cells);
-      if (!e) e = need(&offsets_, &offsets_cap_, &plan->offsets_need, cells);
-      if (!e) e = need(&partial_, &partial_cap_, &plan->partial_need, cells / 4096 + 2);
+      // Twice the records where the arena allows, so that fixed-capacity
+      // buckets keep their margin.
+      int e = grow(to, std::max<uint64_t>(Crow * rb, std::min<uint64_t>(2 * Crow * rb, fixed_ ? 0 : region_cap())));
+      if (!e) e = need(&segs_[sidx ^ 1], &segs_cap_[sidx ^ 1], &plan->segs_need[sidx ^ 1], nout + 1);
       if (e) return e;
-      uint64_t* spare = row_counters(0) + n * kNumCounters;
-      UbCount uc{};
-      bool fuse = false;
-      // (Not on a run-mode level: a row whose only level reads runs reaches
-      // about W / 2^CB + 2 merge workgroups per segment, more than the LDS
-      // counters hold at the default digit width; such rows use count_ub.)
-      if (count_ub && li + 1 == pd.size() && !rn) {
-        const uint64_t nfinal = nseg << dk;
-        const uint32_t W = (uint32_t)std::min<uint64_t>(nfinal, merge_grid);
-        const uint32_t per = (uint32_t)((nfinal + W - 1) / W);
-        if (ub_fits(dk, per, CB)) {
-          const size_t words = ((size_t)W << CB) / 2 + 1;
-          e = need(&ubh_, &ubh_cap_, &plan->ubh_need, words);
-          if (e) return e;
-          LAUNCH("fill", 8.0 * words, launch_fill(ubh_, words, 0, stream_));
-          uc = UbCount{(uint32_t*)ubh_, W, per, CB, gm};
-          fuse = true;
-          ub_W = W;
-        }
+      uint64_t out_cap = 0;
+      if (eager) {
+        // Fixed capacity when each bucket's share of the region exceeds its
+        // mean fill by 10 standard deviations of a uniform hash plus 64.
+        const uint64_t cap = (region_bytes_[to] / rb) / nout;
+        const double mu = (double)Crow / (double)nout;
+        if ((double)cap >= mu + 10.0 * std::sqrt(mu) + 64.0) out_cap = std::min<uint64_t>(cap, (uint64_t)1 << 40);
+        if (spec.force & kForceExactLevels) out_cap = 0;
+        if (out_cap && (spec.force & kForceLevelOverflow)) out_cap = std::max<uint64_t>(1, (uint64_t)(mu / 2));
+      } else {
+        out_cap = (*pcap)[li];
       }
-      if (fuse)
-        LAUNCH("part_hist_ub", 8.0 * Crow,
-               launch_part_hist(LK, region_[cur], segs_[sidx], (uint32_t)nseg, chunks, 64 - used, dk, g.key_bits,
-                                rn, hist_, &uc, stream_));
-      else
+      for (int attempt = 0; attempt < 2; ++attempt) {
+        if (out_cap) {
+          plan->region_need = std::max<size_t>(plan->region_need, nout * out_cap * rb);
+          LAUNCH("fill", 8.0 * nout, launch_fill(segs_[sidx ^ 1], nout, 0, stream_));
+          FanSum fs{};
+          const bool with_fan = fan && last_level;
+          if (with_fan) {
+            const uint32_t W = (uint32_t)std::min<uint64_t>(nout, merge_grid);
+            e = need(&fan_, &fan_cap_, &plan->fan_need, 3 * (size_t)W + 2);
+            if (e) return e;
+            LAUNCH("fill", 8.0 * W, launch_fill(fan_, W, 0, stream_));
+            fs = FanSum{fan_, rowc + kFanout, W, (uint32_t)((nout + W - 1) / W), gm};
+            fan_W = W;
+          }
+          LAUNCH(rn ? "part_scatter_runs" : "part_scatter", 2.0 * R * Crow,
+                 launch_part_scatter(LK, region_[cur], region_[to], bk, chunks, 64 - used, dk, g.key_bits, rn,
+                                     segs_[sidx ^ 1], out_cap, region_bytes_[to] / rb, spare,
+                                     with_fan ? &fs : nullptr, stream_));
+          if (eager) {
+            e = read_block(spare);
+            if (e) return e;
+            if (h_counters_[kRetry] & 32) {
+              // A bucket outgrew its capacity: clear the flag and repeat
+              // the level exactly (its input is intact).
+              LAUNCH("fill", 8.0, launch_fill(spare + kRetry, 1, 0, stream_));
+              if (with_fan) LAUNCH("fill", 8.0, launch_fill(rowc + kFanout, 1, 0, stream_));
+              out_cap = 0;
+              continue;
+            }
+          }
+          fan_ready = with_fan;
+          break;
+        }
+        // Exact level: histogram, scan, scatter at the scanned offsets.
+        const size_t cells = (size_t)nout * chunks;
+        e = need(&hist_, &hist_cap_, &plan->hist_need, cells);
+        if (!e) e = need(&offsets_, &offsets_cap_, &plan->offsets_need, cells);
+        if (!e) e = need(&partial_, &partial_cap_, &plan->partial_need, cells / 4096 + 2);
+        if (e) return e;
         LAUNCH(rn ? "part_hist_runs" : "part_hist", 8.0 * Crow,
-               launch_part_hist(LK, region_[cur], segs_[sidx], (uint32_t)nseg, chunks, 64 - used, dk, g.key_bits,
-                                rn, hist_, nullptr, stream_));
-      ub_fused = fuse;
-      LAUNCH("scan", 12.0 * cells,
-             launch_scan(hist_, (uint32_t)(nseg << dk), chunks, 1, offsets_, segs_[sidx ^ 1], partial_,
-                         rn ? exact : spare, stream_));
-      LAUNCH(rn ? "part_scatter_runs" : "part_scatter", 2.0 * R * Crow,
-             launch_part_scatter(LK, region_[cur], region_[to], segs_[sidx], (uint32_t)nseg, chunks, 64 - used, dk,
-                                 g.key_bits, rn, offsets_, region_bytes_[to] / rb, spare, stream_));
+               launch_part_hist(LK, region_[cur], bk, chunks, 64 - used, dk, g.key_bits, rn, hist_, stream_));
+        LAUNCH("scan", 12.0 * cells,
+               launch_scan(hist_, (uint32_t)nout, chunks, 1, offsets_, segs_[sidx ^ 1], partial_, spare,
+                           stream_));
+        LAUNCH(rn ? "part_scatter_runs" : "part_scatter", 2.0 * R * Crow,
+               launch_part_scatter(LK, region_[cur], region_[to], bk, chunks, 64 - used, dk, g.key_bits, rn,
+                                   offsets_, 0, region_bytes_[to] / rb, spare, nullptr, stream_));
+        break;
+      }
+      if (eager) (*pcap)[li] = out_cap;
       cur = to;
       sidx ^= 1;
-      nseg <<= dk;
+      bk = Buckets{segs_[sidx], out_cap, (uint32_t)nout};
       levels += 1;
       rn = nullptr;
     }
@@ -1020,9 +1064,9 @@ int Engine::pipeline(const RunSpec& spec, Plan* plan, bool eager, RunOut* out, s
            launch_scan(hist_, D1, (uint32_t)tiles, 1, offsets_, segs_[0], partial_, c0, stream_));
     LAUNCH("expand_scatter", R * (S0 + C),
            launch_expand_scatter(LK, skeys, sres, S0, S0, g, pr, d1, (uint32_t)tiles, per_tile, offsets_,
-                                 region_[0], region_bytes_[0] / rb, row_counters(0) + n * kNumCounters,
-                                 stream_));
-    rc = partition_to_final(0, d1, pd, C, nullptr, nullptr, f + 1 < n && C >= kUpperBoundRows);
+                                 region_[0], region_bytes_[0] / rb, spare, stream_));
+    rc = partition_to_final(0, Buckets{segs_[0], 0, D1}, d1, pd, &plan->pcap_first, C, nullptr, c0,
+                            f + 1 < n && C >= kUpperBoundRows);
     if (rc) return rc;
   }
   g.orbit_row = 0;
@@ -1043,128 +1087,166 @@ int Engine::pipeline(const RunSpec& spec, Plan* plan, bool eager, RunOut* out, s
     const bool last = row + 1 == n;
     uint64_t C_next = 0;
     if (C) {
-      const uint32_t W = eager ? (uint32_t)std::min<uint64_t>(nseg, merge_grid) : rp.W;
+      const uint32_t W = eager ? (uint32_t)std::min<uint64_t>(bk.nseg, merge_grid) : rp.W;
       rp.W = W;
       if (last) {
         LAUNCH("merge_last", R * C,
-               launch_merge_expand(kMergeLast, LK, region_[cur], segs_[sidx], (uint32_t)nseg, W, g, pr, 0,
-                                   nullptr, nullptr, nullptr, 0, final_mask, ssum, nslice / L, cr, table_slots,
-                                   stream_));
+               launch_merge_expand(kMergeLast, LK, region_[cur], bk, W, g, pr, 0, nullptr, nullptr, nullptr,
+                                   nullptr, 0, final_mask, ssum, nslice / L, cr, table_slots, stream_));
       } else {
-        // COUNT: successor digits of the merged states at the widest digit
-        // the SCATTER may write (coarsened below once the total is known).
-        // Large rows count the successors of every record instead (an upper
-        // bound per digit and workgroup, no LDS merge); the SCATTER then
-        // leaves a gap after each of its runs and the first partition level
-        // reads the runs.
+        // How the SCATTER finds where to write (rows below 2^20 candidates
+        // count the merged states' successors exactly; larger rows write
+        // capped runs sized from the fanout sums, or, when those are not
+        // available or a run overflowed, runs sized by an upper-bound count
+        // of every record's successors).
         const uint32_t DF = 1u << CB;
-        const bool ub = eager ? C >= kUpperBoundRows : rp.upper_bound;
-        rp.upper_bound = ub;
-        const bool fused = ub && ub_fused;
-        if (ub_fused && (!ub || ub_W != W)) {
-          // A broken plan, not a size problem: fail rather than let the
-          // batched form or the server split the group and retry.
-          *err = "internal: fused upper-bound COUNT does not match row " + std::to_string(row + 1);
+        const int nsidx = sidx ^ 1;
+        const int to = cur ^ 1;
+        int mode = rp.mode;
+        uint64_t fan_total = 0;
+        if (eager) {
+          mode = C < kUpperBoundRows ? RowPlan::kExact : (fan_ready && fan_W == W ? RowPlan::kCapped : RowPlan::kUpper);
+          if (mode == RowPlan::kCapped && (spec.force & kForceUpperRuns)) mode = RowPlan::kUpper;
+          if (mode == RowPlan::kCapped) {
+            rc = read_block(cr);
+            if (rc) return rc;
+            fan_total = h_counters_[kFanout];
+            if (total_bits(fan_total) < 2) mode = RowPlan::kExact;  // few or no successors
+          }
+        } else if (mode == RowPlan::kCapped && !(fan_ready && fan_W == W)) {
+          *err = "internal: replayed fanout sums do not match row " + std::to_string(row + 1);
           return kRcInternal;
         }
-        ub_fused = false;
-        const int nsidx = sidx ^ 1;
+        fan_ready = false;
         rc = need(&hist_, &hist_cap_, &plan->hist_need, (size_t)DF * W);
         if (!rc) rc = need(&offsets_, &offsets_cap_, &plan->offsets_need, (size_t)DF * W);
         if (!rc) rc = need(&partial_, &partial_cap_, &plan->partial_need, ((size_t)DF * W) / 4096 + 2);
         if (!rc) rc = need(&segs_[nsidx], &segs_cap_[nsidx], &plan->segs_need[nsidx], (size_t)DF + 1);
-        if (!rc && ub) rc = need(&run_start_, &run_start_cap_, &plan->run_start_need, (size_t)DF * W);
-        if (!rc && ub) rc = need(&run_len_, &run_len_cap_, &plan->run_len_need, (size_t)DF * W);
+        if (!rc) rc = need(&run_start_, &run_start_cap_, &plan->run_start_need, (size_t)DF * W);
+        if (!rc) rc = need(&run_len_, &run_len_cap_, &plan->run_len_need, (size_t)DF * W);
         if (rc) return rc;
-        // SCATTER write offsets, taken only after every buffer above is sized
-        // (a growing buffer moves).
-        uint64_t* const wpos = ub ? run_start_ : offsets_;
         uint64_t* cn = row_counters(row + 1);
-        // COUNT histogram: from the last partition level (fused) or this pass.
-        uint32_t* const chist = fused ? (uint32_t*)ubh_ : hist_;
-        if (ub && !fused)
-          LAUNCH("count_ub", 8.0 * C,
-                 launch_count_ub(LK, region_[cur], segs_[sidx], (uint32_t)nseg, W, g, CB, hist_, stream_));
-        else if (!ub)
-          LAUNCH("merge_count", 8.0 * C,
-                 launch_merge_expand(kMergeCount, LK, region_[cur], segs_[sidx], (uint32_t)nseg, W, g, pr,
-                                     CB, hist_, nullptr, nullptr, 0, 0, nullptr, 0, cr, table_slots, stream_));
-        LAUNCH("scan", 12.0 * DF * W, launch_scan(chist, DF, W, 1, wpos, segs_[nsidx], partial_, cn, stream_));
-        uint64_t C_count = 0;  // candidates the SCATTER may write (exact, or the upper bound)
-        if (eager) {
-          HIP_TRY(hipMemcpyAsync(h_counters_, cn, sizeof(uint64_t) * kNumCounters, hipMemcpyDeviceToHost,
-                                 stream_));
-          SYNC_AND_RESOLVE();
-          C_count = h_counters_[kCandidates];
-          rp.C_count = C_count;
-        } else {
-          C_count = rp.C_count;
-        }
-        out->peak_write_bound = std::max(out->peak_write_bound, C_count);
-        if (C_count) {
-          unsigned d1 = rp.d1;
-          std::vector<unsigned> pd = rp.pdigits;
-          if (eager) {
-            const unsigned B = total_bits(C_count);
-            d1 = std::min<unsigned>(B, CB);
-            // Upper-bound rows need at least one partition level to close
-            // the gaps between the SCATTER's runs (their levels are planned
-            // from the exact count once the SCATTER has run).
-            if (ub) d1 = B ? std::min<unsigned>(B - 1, CB) : 0;
-            pd = split_bits(B, d1);
-            rp.d1 = d1;
-            rp.pdigits = pd;
-          }
-          if (d1 < CB)
+        unsigned d1 = rp.d1;
+        uint64_t C_count = rp.C_count;  // records the SCATTER's layout spans
+        for (int attempt = 0; attempt < 3; ++attempt) {
+          if (mode == RowPlan::kCapped) {
+            if (eager) {
+              const unsigned B = total_bits(fan_total);
+              d1 = std::min<unsigned>(B - 1, CB);
+            }
+            LAUNCH("caps", 8.0 * W,
+                   launch_caps(fan_, W, d1, (spec.force & kForceCapOverflow) ? 6 : 0, fan_ + W, fan_ + 2 * W,
+                               run_start_, cr, stream_));
+            if (eager) {
+              rc = read_block(cr);
+              if (rc) return rc;
+              C_count = h_counters_[kCapExtent];
+              if (C_count * rb > std::max<uint64_t>(region_bytes_[to], region_cap())) {
+                mode = RowPlan::kUpper;  // the caps do not fit: size the runs exactly
+                continue;
+              }
+            }
+          } else if (mode == RowPlan::kUpper) {
+            LAUNCH("count_ub", 8.0 * C, launch_count_ub(LK, region_[cur], bk, W, g, CB, hist_, stream_));
             LAUNCH("scan", 12.0 * DF * W,
-                   launch_scan(chist, 1u << d1, W, 1u << (CB - d1), wpos, segs_[nsidx], partial_, cn, stream_));
-          const int to = cur ^ 1;
+                   launch_scan(hist_, DF, W, 1, run_start_, segs_[nsidx], partial_, cn, stream_));
+            if (eager) {
+              rc = read_block(cn);
+              if (rc) return rc;
+              C_count = h_counters_[kCandidates];
+              const unsigned B = total_bits(C_count);
+              d1 = B ? std::min<unsigned>(B - 1, CB) : 0;
+            }
+            if (d1 && d1 < CB)
+              LAUNCH("scan", 12.0 * DF * W,
+                     launch_scan(hist_, 1u << d1, W, 1u << (CB - d1), run_start_, segs_[nsidx], partial_,
+                                 cn, stream_));
+          } else {
+            LAUNCH("merge_count", 8.0 * C,
+                   launch_merge_expand(kMergeCount, LK, region_[cur], bk, W, g, pr, CB, hist_, nullptr, nullptr,
+                                       nullptr, 0, 0, nullptr, 0, cr, table_slots, stream_));
+            LAUNCH("scan", 12.0 * DF * W,
+                   launch_scan(hist_, DF, W, 1, offsets_, segs_[nsidx], partial_, cn, stream_));
+            if (eager) {
+              rc = read_block(cn);
+              if (rc) return rc;
+              C_count = h_counters_[kCandidates];
+              d1 = std::min<unsigned>(total_bits(C_count), CB);
+            }
+            if (C_count && d1 < CB)
+              LAUNCH("scan", 12.0 * DF * W,
+                     launch_scan(hist_, 1u << d1, W, 1u << (CB - d1), offsets_, segs_[nsidx], partial_,
+                                 cn, stream_));
+          }
+          if (eager) out->peak_write_bound = std::max(out->peak_write_bound, C_count);
+          if (!C_count) break;
+          const bool runs_out = mode != RowPlan::kExact;
           rc = grow(to, C_count * rb);
           if (rc) return rc;
           LAUNCH("merge_scatter", R * (C + C_count),
-                 launch_merge_expand(kMergeScatter, LK, region_[cur], segs_[sidx], (uint32_t)nseg, W, g, pr, d1,
-                                     ub ? run_len_ : nullptr, wpos, region_[to], region_bytes_[to] / rb, final_mask,
-                                     ssum, nslice / L, cr, table_slots, stream_));
-          sidx = nsidx;
-          // The exact count: from COUNT, or (upper-bound rows) what the
-          // SCATTER wrote, which also corrects its profiled bytes.
-          C_next = C_count;
-          if (ub) {
-            if (eager) {
-              HIP_TRY(hipMemcpyAsync(h_counters_, cr, sizeof(uint64_t) * kNumCounters, hipMemcpyDeviceToHost,
+                 launch_merge_expand(kMergeScatter, LK, region_[cur], bk, W, g, pr, d1,
+                                     runs_out ? run_len_ : nullptr, runs_out ? run_start_ : offsets_,
+                                     mode == RowPlan::kCapped ? fan_ + W : nullptr, region_[to],
+                                     region_bytes_[to] / rb, final_mask, ssum, nslice / L, cr, table_slots,
                                      stream_));
-              HIP_TRY(hipStreamSynchronize(stream_));
-              C_next = h_counters_[kWritten];
-              if (profiling_ && !pending_.empty() && pending_.back().kernel == kernel_id("merge_scatter"))
-                pending_.back().bytes = R * (double)(C + C_next);
-              SYNC_AND_RESOLVE();
-              pd = split_bits(std::max(total_bits(C_next), d1 + 1), d1);
-              rp.pdigits = pd;
-            } else {
-              C_next = rp.C_next;
+          if (eager) {
+            HIP_TRY(hipMemcpyAsync(h_counters_, cr, sizeof(uint64_t) * kNumCounters, hipMemcpyDeviceToHost,
+                                   stream_));
+            HIP_TRY(hipStreamSynchronize(stream_));
+            // Runs with gaps: the SCATTER's algorithmic bytes are its exact
+            // writes, known only now.
+            if (profiling_ && runs_out && !pending_.empty() && pending_.back().kernel == kernel_id("merge_scatter"))
+              pending_.back().bytes = R * (double)(C + h_counters_[kWritten]);
+            SYNC_AND_RESOLVE();
+            if (h_counters_[kRetry] & 16) {
+              // A capped run overflowed: forget this SCATTER's statistics
+              // and repeat it with upper-bound offsets (its input is intact).
+              LAUNCH("fill", 8.0 * kNumCounters, launch_fill(cr, kNumCounters, 0, stream_));
+              mode = RowPlan::kUpper;
+              continue;
             }
+            C_next = runs_out ? h_counters_[kWritten] : C_count;
+          } else {
+            C_next = rp.C_next;
+          }
+          break;
+        }
+        rp.mode = mode;
+        rp.d1 = d1;
+        rp.C_count = C_count;
+        if (C_count && C_next) {
+          sidx = nsidx;
+          std::vector<unsigned> pd = rp.pdigits;
+          if (eager) {
+            // Levels after the SCATTER's digit: runs with gaps need at least
+            // one level to close them.
+            const unsigned B = total_bits(C_next);
+            pd = split_bits(mode != RowPlan::kExact ? std::max(B, d1 + 1) : B, d1);
+            rp.pdigits = pd;
           }
           rp.C_next = C_next;
           const Runs runs{run_start_, run_len_, W};
-          rc = partition_to_final(to, d1, pd, C_next, ub ? &runs : nullptr, cn,
+          rc = partition_to_final(to, Buckets{segs_[sidx], 0, 1u << d1}, d1, pd, &rp.pcap, C_next,
+                                  mode != RowPlan::kExact ? &runs : nullptr, cn,
                                   row + 2 < n && C_next >= kUpperBoundRows);
           if (rc) return rc;
         } else {
           C_next = 0;
           rp.C_next = 0;
-          // No successor survives: merge for the statistics only.
-          LAUNCH("merge_last", R * C,
-                 launch_merge_expand(kMergeLast, LK, region_[cur], segs_[sidx], (uint32_t)nseg, W, g, pr, 0,
-                                     nullptr, nullptr, nullptr, 0, final_mask, ssum, nslice / L, cr, table_slots,
-                                     stream_));
+          if (!C_count) {
+            // No successor survives: merge for the statistics only.
+            LAUNCH("merge_last", R * C,
+                   launch_merge_expand(kMergeLast, LK, region_[cur], bk, W, g, pr, 0, nullptr, nullptr, nullptr,
+                                       nullptr, 0, final_mask, ssum, nslice / L, cr, table_slots, stream_));
+          }
         }
       }
       if (eager) {
-        HIP_TRY(hipMemcpyAsync(h_counters_, cr, sizeof(uint64_t) * kNumCounters, hipMemcpyDeviceToHost, stream_));
-        SYNC_AND_RESOLVE();
+        rc = read_block(cr);
+        if (rc) return rc;
         if (h_counters_[kError] & 4) return kRcPackedOverflow;
-        if (h_counters_[kError]) {
-          *err = error_flag_text(h_counters_[kError], " in row " + std::to_string(row + 1));
+        if (h_counters_[kError] || h_counters_[kRetry]) {
+          *err = error_flag_text(h_counters_[kError] | h_counters_[kRetry], " in row " + std::to_string(row + 1));
           return kRcInternal;
         }
         std::vector<uint64_t> w(L);
@@ -1186,7 +1268,7 @@ int Engine::pipeline(const RunSpec& spec, Plan* plan, bool eager, RunOut* out, s
     }
     if (eager) {
       lo.levels = C ? levels : 0;
-      lo.buckets = C ? (uint32_t)nseg : 0;
+      lo.buckets = C ? bk.nseg : 0;
       lo.device_bytes = arena_bytes() + hist_cap_ * 4 + offsets_cap_ * 8 + (segs_cap_[0] + segs_cap_[1]) * 8;
       lo.elapsed_s = std::chrono::duration<double>(Clock::now() - rt0).count();
       out->layers.push_back(lo);
@@ -1199,6 +1281,7 @@ int Engine::pipeline(const RunSpec& spec, Plan* plan, bool eager, RunOut* out, s
     C = C_next;
     matched = C_next;
   }
+
   if (!eager) return 0;
 
   if (ev_start_) {
@@ -1214,11 +1297,11 @@ int Engine::pipeline(const RunSpec& spec, Plan* plan, bool eager, RunOut* out, s
   }
   // The partition and expansion kernels flag capacity violations in the
   // spare counter row n.
-  std::vector<uint64_t> spare(kNumCounters);
-  HIP_TRY(hipMemcpy(spare.data(), row_counters(n), sizeof(uint64_t) * kNumCounters, hipMemcpyDeviceToHost));
-  if (spare[kError] & 4) return kRcPackedOverflow;
-  if (spare[kError]) {
-    *err = error_flag_text(spare[kError], " (candidate buffers)");
+  std::vector<uint64_t> sp(kNumCounters);
+  HIP_TRY(hipMemcpy(sp.data(), spare, sizeof(uint64_t) * kNumCounters, hipMemcpyDeviceToHost));
+  if (sp[kError] & 4) return kRcPackedOverflow;
+  if (sp[kError] || sp[kRetry]) {
+    *err = error_flag_text(sp[kError] | sp[kRetry], " (candidate buffers)");
     return kRcInternal;
   }
   // Final v1/v2 contraction: the merges summed the residues of the states
@@ -1265,8 +1348,9 @@ int Engine::finish(const RunSpec& spec, const Plan& plan, RunOut* out, std::stri
   HIP_TRY(hipMemcpy(cnt.data(), counters_rows_, cnt.size() * 8, hipMemcpyDeviceToHost));
   for (unsigned row = 0; row <= n; ++row)
     if (cnt[(size_t)row * kNumCounters + kError] & 4) return kRcPackedOverflow;
-  if (cnt[(size_t)n * kNumCounters + kError]) {
-    *err = error_flag_text(cnt[(size_t)n * kNumCounters + kError], " (candidate buffers, replayed size plan)");
+  if (cnt[(size_t)n * kNumCounters + kError] || cnt[(size_t)n * kNumCounters + kRetry]) {
+    *err = error_flag_text(cnt[(size_t)n * kNumCounters + kError] | cnt[(size_t)n * kNumCounters + kRetry],
+                           " (candidate buffers, replayed size plan)");
     return kRcInternal;
   }
   out->record_bytes = (uint32_t)record_bytes(packed_records(spec) ? kPackedRecords : L);
@@ -1288,11 +1372,11 @@ int Engine::finish(const RunSpec& spec, const Plan& plan, RunOut* out, std::stri
     lo.candidates = (u128)n * S_in;
     lo.matched = matched;
     lo.completed = C;
-    if (c[kError]) {
-      *err = error_flag_text(c[kError], " in row " + std::to_string(row + 1) + " (replayed size plan)");
+    if (c[kError] || c[kRetry]) {
+      *err = error_flag_text(c[kError] | c[kRetry], " in row " + std::to_string(row + 1) + " (replayed size plan)");
       return kRcInternal;
     }
-    if (row + 1 < n && C && cnt[(size_t)(row + 1) * kNumCounters + kCandidates] != rp.C_next) {
+    if (row + 1 < n && C && rp.C_next && c[kWritten] != rp.C_next) {
       *err = "replayed size plan does not match the device counts";
       return kRcInternal;
     }
@@ -1527,14 +1611,15 @@ int Engine::materialized(const RunSpec& spec, RunOut* out, std::string* err) {
         if (!rc) rc = ensure_buffer(&partial_, &partial_cap_, cells / 4096 + 2, err);
         if (rc) return rc;
         LAUNCH("part_hist", 8.0 * C,
-               launch_part_hist(L, region_[cur], segs_[sidx], (uint32_t)nseg, chunks, 64 - used, dk, 0, nullptr,
-                                hist_, nullptr, stream_));
+               launch_part_hist(L, region_[cur], Buckets{segs_[sidx], 0, (uint32_t)nseg}, chunks, 64 - used, dk, 0,
+                                nullptr, hist_, stream_));
         LAUNCH("scan", 12.0 * cells,
                launch_scan(hist_, (uint32_t)(nseg << dk), chunks, 1, offsets_, segs_[sidx ^ 1], partial_, spare,
                            stream_));
         LAUNCH("part_scatter", 2.0 * R * C,
-               launch_part_scatter(L, region_[cur], region_[to], segs_[sidx], (uint32_t)nseg, chunks, 64 - used,
-                                   dk, 0, nullptr, offsets_, region_bytes_[to] / rb, spare, stream_));
+               launch_part_scatter(L, region_[cur], region_[to], Buckets{segs_[sidx], 0, (uint32_t)nseg}, chunks,
+                                   64 - used, dk, 0, nullptr, offsets_, 0, region_bytes_[to] / rb, spare, nullptr,
+                                   stream_));
         cur = to;
         sidx ^= 1;
         nseg <<= dk;
@@ -1546,8 +1631,9 @@ int Engine::materialized(const RunSpec& spec, RunOut* out, std::string* err) {
       if (rc) return rc;
       const uint32_t W = (uint32_t)std::min<uint64_t>(nseg, merge_grid);
       LAUNCH("merge_emit", R * 2.0 * C,
-             launch_merge_expand(kMergeEmit, L, region_[cur], segs_[sidx], (uint32_t)nseg, W, g0, pr, 0, nullptr,
-                                 nullptr, front_, C, final_mask, nullptr, 0, cr, table_slots, stream_));
+             launch_merge_expand(kMergeEmit, L, region_[cur], Buckets{segs_[sidx], 0, (uint32_t)nseg}, W, g0, pr, 0,
+                                 nullptr, nullptr, nullptr, front_, C, final_mask, nullptr, 0, cr, table_slots,
+                                 stream_));
       rc = read_counters(cr);
       if (rc) return rc;
       if (h[kError]) {

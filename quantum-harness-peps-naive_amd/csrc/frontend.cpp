@@ -134,10 +134,12 @@ int resolve_options(unsigned n, const peps_options* o, Resolved* r, std::string*
   }
   r->site_counters = o->site_counters;
   const uint32_t fmt = o->record_format & 0xffu, wbits = (o->record_format >> 8) & 0xffu;
-  if (fmt > PEPS_RECORDS_PACKED || (o->record_format >> 16)) {
-    *err = "record_format must be 0 (auto), 1 (lane records) or 2 (packed), with a packed weight width in bits 8..15";
+  if (fmt > PEPS_RECORDS_PACKED || (o->record_format >> 20)) {
+    *err = "record_format must be 0 (auto), 1 (lane records) or 2 (packed), with a packed weight width in "
+           "bits 8..15 and fallback-path bits in 16..19";
     return PEPS_E_INVALID;
   }
+  r->force = (o->record_format >> 16) & 0xfu;
   r->records = (int)fmt;
   r->packed_weight_bits = wbits ? wbits : kPackedMaxWeightBits;
   return PEPS_OK;
@@ -166,6 +168,7 @@ void apply_resolved(const Resolved& r, RunSpec* spec) {
   spec->budget_bytes = r.budget;
   spec->records = r.records;
   spec->packed_weight_bits = r.packed_weight_bits;
+  spec->force = r.force;
 }
 
 int run_engine(const Resolved& r, RunSpec& spec, RunOut* ro, std::string* err) {

@@ -35,6 +35,11 @@ void fill_result(unsigned n, const RunOut& ro, const std::vector<uint64_t>& prim
                  peps_contraction_result* out, TensorCounters tc = TensorCounters::kCompiled,
                  uint64_t c_entries = 17);
 
+// Share of one candidate buffer that the batched form and the server admit
+// a launch's projected candidates into.  The rest keeps the margin that the
+// fixed-capacity partition buckets need (about 1.8x the records of a row).
+constexpr double kAdmitFraction = 0.55;
+
 // peps_options after validation and defaults.
 struct Resolved {
   std::vector<uint64_t> primes;
@@ -50,6 +55,7 @@ struct Resolved {
   uint32_t site_counters = 0;
   int records = 0;                  // RecordFormat
   uint32_t packed_weight_bits = 52;
+  uint32_t force = 0;               // ForcePath bits (record_format bits 16..19)
 };
 
 // Lane count: the reference plan (src/lib.rs:3859-3886) unless the caller

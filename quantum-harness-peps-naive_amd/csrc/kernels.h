@@ -12,18 +12,22 @@
 //
 // Row pipeline (candidates of row r live in hash-radix buckets):
 //   expand (seed rows only): frontier -> candidates, level-1 buckets
-//   partition             : one more hash-radix level inside each bucket
-//   merge_expand COUNT    : per bucket, LDS-merge keys, count successors of
-//                           the merged states per next-row level-1 digit
-//                           (large rows: every record's successors, an
-//                           upper bound, without the merge -- counted by the
-//                           last partition level's histogram launch, or by
-//                           count_ub where that level cannot hold it)
-//   scan                  : digit-major exclusive scan -> write offsets
+//   partition             : one more hash-radix level inside each bucket,
+//                           into fixed-capacity buckets filled by per-stage
+//                           claims (no histogram pass); the last level also
+//                           sums every record's fanout per merge workgroup
+//                           (an exact histogram + scan level is the fallback)
+//   caps (large rows)     : per merge workgroup, a cap per digit on its
+//                           SCATTER run from its fanout sum
+//   merge_expand COUNT    : (smaller rows) per bucket, LDS-merge keys, count
+//                           successors of the merged states per next-row
+//                           level-1 digit, then a digit-major scan
 //   merge_expand SCATTER  : merge with weights, layer statistics, write the
-//                           successors into the next row's level-1 buckets
-//                           (after count_ub: runs with gaps, which the first
-//                           partition level reads in run mode)
+//                           successors into the next row's level-1 runs
+//                           (capped runs have gaps, which the first
+//                           partition level reads in run mode; a run over
+//                           its cap makes the engine count exact upper
+//                           bounds, count_ub, and repeat the SCATTER)
 //   merge_expand LAST     : last row: merge, statistics, final v1/v2 sum
 #pragma once
 
@@ -77,7 +81,11 @@ enum Counter : int {
   kSiteMatched = 13, // sitewise: C entries applied (= entries examined through the input index)
   kEmitted = 14,     // EMIT merge: merged states written to the frontier
   kWritten = 15,     // SCATTER merge: successors written (the next row's exact candidates)
-  kNumCounters = 16
+  kFanout = 16,      // last partition level: sum of every record's legal positions
+  kCapExtent = 17,   // capped SCATTER: records its runs span (digits x summed caps)
+  kRetry = 18,       // bit 4: a capped SCATTER run overflowed; bit 5: a fixed-capacity
+                     // bucket overflowed (the engine repeats that step with exact offsets)
+  kNumCounters = 19
 };
 
 // COUNT / SCATTER expand the merged states into the next row; LAST only sums;
@@ -143,68 +151,94 @@ hipError_t launch_scan(const uint32_t* hist, uint32_t D, uint32_t W, uint32_t gr
                        uint64_t* offsets, uint64_t* segs, uint64_t* partial, uint64_t* counters,
                        hipStream_t stream);
 
+// The buckets of one level of a row: contiguous (cap == 0: segs holds
+// nseg + 1 bounds) or fixed capacity (cap > 0: bucket b is the records
+// [b * cap, b * cap + min(segs[b], cap)), segs[b] its fill cursor).
+struct Buckets {
+  const uint64_t* segs;
+  uint64_t cap;
+  uint32_t nseg;
+};
+
 // One more hash-radix digit (bits [shift, shift + digit_bits) of the key mix)
-// inside each of the nseg segments of `in`, written to `out`; the scan's segs
-// output is the nseg * 2^digit_bits + 1 new segment bounds.  Every segment is
-// processed as `chunks` ranges (partition_chunks); hist and offsets need
-// nseg * 2^digit_bits * chunks entries, partial that / 4096 + 2.  A write at
-// or beyond `cap` sets bit 1 of counters[kError].
+// inside each of the nseg input buckets, written to `out`.  Every bucket is
+// processed as `chunks` ranges (partition_chunks).
+//  * Exact (out_cap == 0): launch_part_hist counts every (bucket, digit,
+//    chunk) into hist, launch_scan(hist, nseg << digit_bits, chunks, 1,
+//    offsets, subsegs, partial, counters) turns the counts into write
+//    offsets and the next level's contiguous bounds, and launch_part_scatter
+//    writes at those offsets.  hist and offsets need nseg * 2^digit_bits *
+//    chunks entries, partial that / 4096 + 2.
+//  * Fixed capacity (out_cap > 0): no histogram pass.  Output bucket
+//    (s, d) is [(s * D + d) * out_cap, ... + out_cap) and `offsets` holds
+//    its fill cursor (nseg * 2^digit_bits entries, zeroed by the caller);
+//    each LDS stage claims its digit runs with one atomic add per digit.  A
+//    bucket that would pass out_cap sets bit 5 of counters[kError] and
+//    writes nothing beyond it (the engine repeats the level exactly).
+// A write at or beyond `cap` (the output region's records) sets bit 1.
 constexpr uint64_t kPartChunk = 65536;     // target records per partition workgroup
 constexpr uint64_t kMaxPartChunks = 4096;
 uint32_t partition_chunks(uint64_t records, uint64_t nseg);
 // Run mode of a partition level (its input is the SCATTER output with
-// upper-bound offsets): segment s is the runs w < per_seg, run (s, w) being
-// the records [start[s * per_seg + w], start[...] + len[s * per_seg + w]).
-// Chunk p takes the runs [per_seg * p / chunks, per_seg * (p + 1) / chunks),
-// at most kMaxChunkRuns of them, so chunks >= ceil(per_seg / kMaxChunkRuns).
+// capped or upper-bound offsets): segment s is the runs w < per_seg, run
+// (s, w) being the records [start[s * per_seg + w], start[...] +
+// len[s * per_seg + w]).  Chunk p takes the runs [per_seg * p / chunks,
+// per_seg * (p + 1) / chunks), at most kMaxChunkRuns of them, so
+// chunks >= ceil(per_seg / kMaxChunkRuns).
 constexpr uint32_t kMaxChunkRuns = 64;
 struct Runs {
   const uint64_t* start;
   const uint32_t* len;
   uint32_t per_seg;
 };
-// Upper-bound COUNT folded into the last partition level of a row (the
-// histogram launch of a contiguous level; not with `runs`): the successor digits (`bits` wide) of every record,
-// counted per merge workgroup of a W-workgroup merge launch that gives
-// workgroup w the final buckets [w * per, (w + 1) * per), added into
-// hist[d * W + w] (zeroed by the caller).  A workgroup counts one segment
-// at a time in kUbSlots LDS counters, one per (workgroup, digit) pair the
-// segment can reach: ub_fits(digit_bits, per, bits) must hold.
-constexpr uint32_t kUbSlots = 1024;
-struct UbCount {
-  uint32_t* hist;
-  uint32_t W, per, bits;
+// Fanout sums folded into the last (fixed-capacity) partition level of a
+// row: every record's number of legal positions in the next row (the
+// successors it yields if it is a merged state; duplicates count once per
+// record, so the sums bound the SCATTER's writes from above) is added to
+// U[w] of the merge workgroup w = (s * 2^digit_bits + d) / per that will own
+// its final bucket (a merge launch of W workgroups, workgroup w owning the
+// final buckets [w * per, (w + 1) * per)), and to *total.  U is zeroed by
+// the caller.
+struct FanSum {
+  uint64_t* U;
+  uint64_t* total;
+  uint32_t W, per;
   RowGeom g;
 };
 // Most merge workgroups whose buckets one segment's 2^digit_bits final
 // buckets can touch.
 inline uint32_t ub_span(uint32_t digit_bits, uint32_t per) { return ((1u << digit_bits) - 1) / per + 2; }
-inline bool ub_fits(uint32_t digit_bits, uint32_t per, uint32_t bits) {
-  return per > 0 && bits >= 1 && bits <= 10 && ((uint64_t)ub_span(digit_bits, per) << bits) <= kUbSlots;
-}
-// The partition is three launches: hist, launch_scan(hist, nseg << digit_bits,
-// chunks, 1, offsets, subsegs, partial, counters), scatter.
-// key_bits: the packed records' key width (ignored for lane records).  runs:
-// run mode (segs unused), or null for contiguous segments.  ub: also the
-// upper-bound COUNT of the buckets this level writes (or null).
-hipError_t launch_part_hist(int lanes, const void* in, const uint64_t* segs, uint32_t nseg, uint32_t chunks,
-                            uint32_t shift, uint32_t digit_bits, uint32_t key_bits, const Runs* runs,
-                            uint32_t* hist, const UbCount* ub, hipStream_t stream);
-hipError_t launch_part_scatter(int lanes, const void* in, void* out, const uint64_t* segs, uint32_t nseg,
-                               uint32_t chunks, uint32_t shift, uint32_t digit_bits, uint32_t key_bits,
-                               const Runs* runs, const uint64_t* offsets, uint64_t cap, uint64_t* counters,
-                               hipStream_t stream);
+// key_bits: the packed records' key width (ignored for lane records).
+// runs: run mode (bk.segs unused), or null.
+hipError_t launch_part_hist(int lanes, const void* in, const Buckets& bk, uint32_t chunks, uint32_t shift,
+                            uint32_t digit_bits, uint32_t key_bits, const Runs* runs, uint32_t* hist,
+                            hipStream_t stream);
+hipError_t launch_part_scatter(int lanes, const void* in, void* out, const Buckets& bk, uint32_t chunks,
+                               uint32_t shift, uint32_t digit_bits, uint32_t key_bits, const Runs* runs,
+                               uint64_t* offsets, uint64_t out_cap, uint64_t cap, uint64_t* counters,
+                               const FanSum* fan, hipStream_t stream);
 
 // Upper-bound COUNT: successor digits of every record of the buckets (no
 // merge), per merge workgroup of a W-workgroup merge launch, hist[d * W + w].
-hipError_t launch_count_ub(int lanes, const void* in, const uint64_t* segs, uint32_t nseg, uint32_t W,
-                           const RowGeom& g, uint32_t digit_bits, uint32_t* hist, hipStream_t stream);
+hipError_t launch_count_ub(int lanes, const void* in, const Buckets& bk, uint32_t W, const RowGeom& g,
+                           uint32_t digit_bits, uint32_t* hist, hipStream_t stream);
+
+// Capped SCATTER runs from the fanout sums U[W] (FanSum): workgroup w may
+// write cap_w = ceil(U_w / D) + 8 sqrt(U_w / D) + 32 successors per digit
+// (D = 2^digit_bits) into the run starting at run_start[d * W + w] =
+// d * T + (cap_0 + .. + cap_{w-1}), T = sum of the caps.  capw needs W and
+// pre W + 1 entries, run_start D * W; counters[kCapExtent] = D * T.
+// shrink > 0 divides the caps by 2^shrink (tests: forces the overflow path).
+hipError_t launch_caps(const uint64_t* U, uint32_t W, uint32_t digit_bits, uint32_t shrink, uint64_t* capw,
+                       uint64_t* pre, uint64_t* run_start, uint64_t* counters, hipStream_t stream);
 
 // Fused merge of row r's buckets with the expansion into row r+1.
 // COUNT: hist[(1<<kFineBits)][W] of successor digits.  SCATTER: successors
 // written to out (capacity `cap` records) at offsets[(1<<digit_bits)][W]; if
 // hist is given, the number written per (digit, workgroup) goes there (the
-// run lengths when the offsets are upper bounds).
+// run lengths when the offsets are capped or upper bounds).  capw (or null):
+// workgroup w writes at most capw[w] successors per digit; more set bit 4 of
+// counters[kRetry].
 // LAST: no expansion.  EMIT (lane records only): the merged states are
 // written to `out` as a frontier (keys[cap] then residues[l * cap + i]),
 // positions counted in counters[kEmitted].  Merged states with (key & final_mask) == final_mask
@@ -212,12 +246,11 @@ hipError_t launch_count_ub(int lanes, const void* in, const uint64_t* segs, uint
 // into slice_sums[slice * L + l] (slice = key bits above 3N).
 // table_slots selects the LDS table (2048 or 4096 slots); a bucket of up to
 // merge_cap(table_slots) records is merged in one pass.
-hipError_t launch_merge_expand(int mode, int lanes, const void* in, const uint64_t* segs,
-                               uint32_t nseg, uint32_t W, const RowGeom& g, const Primes& pr,
-                               uint32_t digit_bits, uint32_t* hist, const uint64_t* offsets,
-                               void* out, uint64_t cap, uint64_t final_mask, uint64_t* slice_sums,
-                               uint64_t nslices, uint64_t* counters, uint32_t table_slots,
-                               hipStream_t stream);
+hipError_t launch_merge_expand(int mode, int lanes, const void* in, const Buckets& bk, uint32_t W,
+                               const RowGeom& g, const Primes& pr, uint32_t digit_bits, uint32_t* hist,
+                               const uint64_t* offsets, const uint64_t* capw, void* out, uint64_t cap,
+                               uint64_t final_mask, uint64_t* slice_sums, uint64_t nslices, uint64_t* counters,
+                               uint32_t table_slots, hipStream_t stream);
 uint32_t merge_cap(uint32_t table_slots);
 // Workgroups of the SCATTER merge kernel (writing digits of up to
 // `digit_bits`) that fit on one CU at once.

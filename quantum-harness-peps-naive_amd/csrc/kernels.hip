@@ -118,6 +118,22 @@ __device__ __forceinline__ void lds_barrier() {
   asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
 }
 
+// Record range of bucket b of a level: contiguous buckets (cap == 0) are
+// [segs[b], segs[b + 1]); fixed-capacity buckets are [b * cap, b * cap +
+// min(segs[b], cap)), segs[b] being the bucket's fill cursor (a cursor past
+// cap means the level overflowed, which its writer flagged).
+__device__ __forceinline__ void bucket_range(const uint64_t* __restrict__ segs, uint64_t cap, uint64_t b,
+                                             uint64_t* lo, uint64_t* hi) {
+  if (cap) {
+    const uint64_t n = segs[b];
+    *lo = b * cap;
+    *hi = *lo + (n < cap ? n : cap);
+  } else {
+    *lo = segs[b];
+    *hi = segs[b + 1];
+  }
+}
+
 // Bijective 64-bit mix (xorshift, odd multiply, xorshift): equal keys share
 // every derived digit and slot; distinct keys spread uniformly over the top
 // bits (radix digits), bits 12-35 (merge-pass class) and the low bits (LDS
@@ -368,8 +384,8 @@ __global__ __launch_bounds__(kBlock) void k_expand_hist(const uint64_t* __restri
 // the LDS table.  hist[d * W + w].
 template <int L>
 __global__ __launch_bounds__(kBlock) void k_count_ub(const Rec<L>* __restrict__ in,
-                                                     const uint64_t* __restrict__ segs, uint32_t nseg,
-                                                     uint32_t per, RowGeom g, uint32_t digit_bits,
+                                                     const uint64_t* __restrict__ segs, uint64_t in_cap,
+                                                     uint32_t nseg, uint32_t per, RowGeom g, uint32_t digit_bits,
                                                      uint32_t* __restrict__ hist) {
   __shared__ uint32_t lh[kExpandDigits];
   const uint32_t D = 1u << digit_bits;
@@ -378,21 +394,86 @@ __global__ __launch_bounds__(kBlock) void k_count_ub(const Rec<L>* __restrict__ 
   lds_barrier();
   const uint32_t b0 = blockIdx.x * per;
   const uint32_t b1 = (b0 + per < nseg) ? b0 + per : nseg;
-  if (b0 < b1) {
-    const uint64_t lo = segs[b0], hi = segs[b1];
+  for (uint32_t b = b0; b < b1; ++b) {
+    uint64_t lo, hi;
+    bucket_range(segs, in_cap, b, &lo, &hi);
     for (uint64_t i = lo + threadIdx.x; i < hi; i += kBlock) {
       const uint64_t key = load_key_stream(in + i, g.key_bits);
       const Board bs = unpack(key, g);
       uint32_t free = legal_positions(bs, g);
       while (free) {
-        const uint32_t b = free & (~free + 1);
-        free ^= b;
-        atomicAdd(&lh[top_digit(successor(key, bs, b, g), digit_bits)], 1u);
+        const uint32_t bit = free & (~free + 1);
+        free ^= bit;
+        atomicAdd(&lh[top_digit(successor(key, bs, bit, g), digit_bits)], 1u);
       }
     }
   }
   lds_barrier();
   for (uint32_t i = threadIdx.x; i < D; i += kBlock) hist[(uint64_t)i * W + blockIdx.x] = lh[i];
+}
+
+// ------------------------------------------------------------------ caps --
+// Capped SCATTER runs: merge workgroup w's successors of digit d go to
+// run_start[d * W + w] = d * T + P_w, at most cap_w of them, where
+// cap_w = ceil(U_w / D) + 8 sqrt(U_w / D) + 32 from the fanout sum U_w of
+// the records w will merge (an upper bound of its successors: duplicates
+// count once per record) and P_w = cap_0 + .. + cap_{w-1}, T = P_W.  One
+// workgroup scans the W caps; counters[kCapExtent] = D * T.
+constexpr int kCapsBlock = 1024;
+// shrink > 0 divides every cap by 2^shrink (a test that forces the overflow
+// path).
+__global__ __launch_bounds__(kCapsBlock) void k_caps_scan(const uint64_t* __restrict__ U, uint32_t W,
+                                                          uint32_t digit_bits, uint32_t shrink,
+                                                          uint64_t* __restrict__ capw, uint64_t* __restrict__ pre,
+                                                          uint64_t* __restrict__ counters) {
+  __shared__ uint64_t wave_tot[kCapsBlock / 64];
+  const uint32_t per = (W + kCapsBlock - 1) / kCapsBlock;
+  const uint32_t lo = threadIdx.x * per;
+  const double D = (double)(1u << digit_bits);
+  uint64_t local = 0;
+  for (uint32_t i = 0; i < per; ++i) {
+    const uint32_t w = lo + i;
+    if (w < W) {
+      const double m = (double)U[w] / D;
+      const uint64_t c = ((uint64_t)__builtin_ceil(m) + (uint64_t)(8.0 * __builtin_sqrt(m)) + 32) >> shrink;
+      capw[w] = c;
+      local += c;
+    }
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  uint64_t x = local;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint64_t y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) wave_tot[wave] = x;
+  __syncthreads();
+  uint64_t before = 0, all = 0;
+  for (int w = 0; w < kCapsBlock / 64; ++w) {
+    before += (w < wave) ? wave_tot[w] : 0;
+    all += wave_tot[w];
+  }
+  uint64_t run = before + x - local;
+  for (uint32_t i = 0; i < per; ++i) {
+    const uint32_t w = lo + i;
+    if (w < W) {
+      pre[w] = run;
+      run += capw[w];
+    }
+  }
+  if (threadIdx.x == 0) {
+    pre[W] = all;
+    counters[kCapExtent] = all << digit_bits;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_caps_expand(const uint64_t* __restrict__ pre, uint32_t W,
+                                                        uint32_t digit_bits, uint64_t* __restrict__ run_start) {
+  const uint64_t T = pre[W];
+  const uint64_t total = (uint64_t)W << digit_bits;
+  for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < total; i += (uint64_t)gridDim.x * kBlock)
+    run_start[i] = (i / W) * T + pre[i % W];
 }
 
 // --------------------------------------------------------- expand scatter --
@@ -741,9 +822,11 @@ __global__ __launch_bounds__(kBlock) void k_scan_apply(const uint32_t* __restric
 // an LDS stage grouped by digit (the next chunk's loads in flight meanwhile).
 // PB: widest digit of the launch; the per-digit LDS arrays are sized 2^PB.
 
-__device__ __forceinline__ void chunk_range(const uint64_t* __restrict__ segs, uint32_t s, uint32_t p,
-                                            uint32_t chunks, uint64_t* a, uint64_t* b) {
-  const uint64_t lo = segs[s], len = segs[s + 1] - lo;
+__device__ __forceinline__ void chunk_range(const uint64_t* __restrict__ segs, uint64_t in_cap, uint32_t s,
+                                            uint32_t p, uint32_t chunks, uint64_t* a, uint64_t* b) {
+  uint64_t lo, hi;
+  bucket_range(segs, in_cap, s, &lo, &hi);
+  const uint64_t len = hi - lo;
   *a = lo + len * p / chunks;
   *b = lo + len * (p + 1) / chunks;
 }
@@ -793,24 +876,14 @@ __device__ __forceinline__ uint64_t run_record(const RunTable& t, uint32_t nr, u
   return t.start[r] + (j - t.pre[r]);
 }
 
-// UB (contiguous levels only, not run mode): the last partition level of a
-// row also does the upper-bound COUNT of k_count_ub.  A record of segment s with digit d lands in final bucket
-// s * D + d, which merge workgroup (s * D + d) / per owns, so its successors'
-// top digits are counted here per (digit, workgroup) instead of by a second
-// read of the buckets.  The (workgroup, digit) pairs a segment can reach fit
-// the kUbSlots LDS counters (checked by the launcher); each chunk's counts
-// are added into uc.hist[digit * W + w], which the caller zeroes.
-template <int L, int PB, bool RUNS, bool UB>
+template <int L, int PB, bool RUNS>
 __global__ __launch_bounds__(kBlock) void k_part_hist(const Rec<L>* __restrict__ in,
-                                                      const uint64_t* __restrict__ segs, uint32_t nseg,
-                                                      uint32_t chunks, uint32_t shift, uint32_t digit_bits,
-                                                      uint32_t kb, Runs rn, uint32_t* __restrict__ hist,
-                                                      UbCount uc) {
-  static_assert(!(RUNS && UB), "the fused COUNT runs on contiguous levels only");
+                                                      const uint64_t* __restrict__ segs, uint64_t in_cap,
+                                                      uint32_t nseg, uint32_t chunks, uint32_t shift,
+                                                      uint32_t digit_bits, uint32_t kb, Runs rn,
+                                                      uint32_t* __restrict__ hist) {
   __shared__ uint32_t cnt[1 << PB];
   __shared__ RunTable rt[1];  // run mode only (unreferenced otherwise)
-  __shared__ uint32_t ul[UB ? kUbSlots : 1];
-  __shared__ uint16_t wmap[UB ? (1 << PB) : 1];
   const uint32_t D = 1u << digit_bits;
   const uint64_t dmask = D - 1;
   const uint64_t total = (uint64_t)nseg * chunks;
@@ -821,14 +894,8 @@ __global__ __launch_bounds__(kBlock) void k_part_hist(const Rec<L>* __restrict__
     if constexpr (RUNS)
       nr = load_runs(rn, s, p, chunks, rt[0]);
     else
-      chunk_range(segs, s, p, chunks, &a, &b);
+      chunk_range(segs, in_cap, s, p, chunks, &a, &b);
     for (uint32_t d = threadIdx.x; d < D; d += kBlock) cnt[d] = 0;
-    uint32_t w0 = 0;
-    if constexpr (UB) {
-      w0 = (s * D) / uc.per;
-      for (uint32_t d = threadIdx.x; d < D; d += kBlock) wmap[d] = (uint16_t)((s * D + d) / uc.per - w0);
-      for (uint32_t i = threadIdx.x; i < kUbSlots; i += kBlock) ul[i] = 0;
-    }
     lds_barrier();
     // Records [a, b): record indices, or (run mode) indices into the
     // chunk's run table.
@@ -848,31 +915,10 @@ __global__ __launch_bounds__(kBlock) void k_part_hist(const Rec<L>* __restrict__
       }
 #pragma unroll
       for (int j = 0; j < kPerThread; ++j)
-        if (k[j] != kEmptyKey) {
-          const uint32_t d = (uint32_t)((mix64(k[j]) >> shift) & dmask);
-          atomicAdd(&cnt[d], 1u);
-          if constexpr (UB) {
-            const uint32_t row = (uint32_t)wmap[d] << uc.bits;
-            const Board bs = unpack(k[j], uc.g);
-            uint32_t free = legal_positions(bs, uc.g);
-            while (free) {
-              const uint32_t bit = free & (~free + 1);
-              free ^= bit;
-              atomicAdd(&ul[row + top_digit(successor(k[j], bs, bit, uc.g), uc.bits)], 1u);
-            }
-          }
-        }
+        if (k[j] != kEmptyKey) atomicAdd(&cnt[(uint32_t)((mix64(k[j]) >> shift) & dmask)], 1u);
     }
     lds_barrier();
     for (uint32_t d = threadIdx.x; d < D; d += kBlock) hist[((uint64_t)s * D + d) * chunks + p] = cnt[d];
-    if constexpr (UB) {
-      const uint32_t dfm = (1u << uc.bits) - 1;
-      for (uint32_t i = threadIdx.x; i < kUbSlots; i += kBlock) {
-        const uint32_t v = ul[i];
-        const uint32_t w = w0 + (i >> uc.bits);
-        if (v && w < uc.W) atomicAdd(&uc.hist[(uint64_t)(i & dfm) * uc.W + w], v);
-      }
-    }
     lds_barrier();
   }
 }
@@ -888,41 +934,60 @@ __global__ __launch_bounds__(kBlock) void k_part_hist(const Rec<L>* __restrict__
 // than four 256-thread workgroups with 2048- or 2560-record stages.
 constexpr int kPartBlock = 512;
 constexpr int kPartLds = 152 * 1024;  // LDS budget of one partition workgroup
-template <int L, int PB, bool RUNS>
+template <int L, int PB, bool FAN>
 constexpr int part_stage() {
   // per staged record: the record and its digit; fixed: count, start and
-  // gpos per digit, the run table and the wave totals
+  // gpos per digit, the run table and the wave totals, and (FAN) the
+  // fanout counters of the merge workgroups one segment reaches
   constexpr int per = (int)sizeof(Rec<L>) + (PB <= 8 ? 1 : 2);
-  constexpr int fixed = (1 << PB) * 16 + 1024;
+  constexpr int fixed = (1 << PB) * 16 + 1024 + (FAN ? ((1 << PB) + 2) * 4 + (1 << PB) * 2 + kPartBlock / 8 : 0);
   return (kPartLds - fixed) / per / kPartBlock * kPartBlock;
 }
 
-template <int L, int PB, bool RUNS>
+template <int L, int PB, bool FAN>
 struct PartStage {
-  Rec<L> rec[part_stage<L, PB, RUNS>()];
-  std::conditional_t<(PB <= 8), uint8_t, uint16_t> digit[part_stage<L, PB, RUNS>()];
+  Rec<L> rec[part_stage<L, PB, FAN>()];
+  std::conditional_t<(PB <= 8), uint8_t, uint16_t> digit[part_stage<L, PB, FAN>()];
   uint32_t count[1 << PB];
   uint32_t start[1 << PB];
   uint64_t gpos[1 << PB];
   uint32_t wave_tot[kPartBlock / 64];
 };
 
-template <int L, int PB, bool RUNS>
+// FIXED: the output buckets have a fixed capacity `out_cap` (bucket (s, d)
+// is [(s * D + d) * out_cap, ... + out_cap)) and a fill cursor each
+// (cursor[s * D + d], zeroed by the caller); every stage claims each
+// digit's run with one atomic add on its cursor, so no histogram pass is
+// needed.  A claim past the capacity writes nothing beyond it and sets bit 5
+// of counters[kError] (the engine then repeats the level with exact
+// offsets).  Otherwise the stage writes at the exact `offsets` of a
+// histogram pass and its scan.
+// FAN: the last partition level of a row also adds, for every record, the
+// number of its legal positions (successors, duplicates counted per record)
+// to fan.U[w] of the merge workgroup w = (s * D + d) / fan.per that will own
+// its final bucket, and to *fan.total.
+template <int L, int PB, bool RUNS, bool FIXED, bool FAN>
 __global__ __launch_bounds__(kPartBlock, 1) void k_part_scatter(const Rec<L>* __restrict__ in,
                                                          Rec<L>* __restrict__ out,
-                                                         const uint64_t* __restrict__ segs, uint32_t nseg,
-                                                         uint32_t chunks, uint32_t shift, uint32_t digit_bits,
-                                                         uint32_t kb, Runs rn, const uint64_t* __restrict__ offsets,
-                                                         uint64_t cap, uint64_t* __restrict__ counters) {
-  constexpr int PS = part_stage<L, PB, RUNS>();
+                                                         const uint64_t* __restrict__ segs, uint64_t in_cap,
+                                                         uint32_t nseg, uint32_t chunks, uint32_t shift,
+                                                         uint32_t digit_bits, uint32_t kb, Runs rn,
+                                                         uint64_t* __restrict__ offsets, uint64_t out_cap,
+                                                         uint64_t cap, uint64_t* __restrict__ counters, FanSum fan) {
+  constexpr int PS = part_stage<L, PB, FAN>();
   constexpr int BT = kPartBlock;
   constexpr int PT = PS / BT;
   static_assert(PS % BT == 0, "stage is whole rounds of the workgroup");
-  __shared__ PartStage<L, PB, RUNS> st;
+  __shared__ PartStage<L, PB, FAN> st;
   __shared__ RunTable rt[1];  // run mode only (unreferenced otherwise)
+  __shared__ uint32_t fl[FAN ? (1 << PB) + 2 : 1];
+  __shared__ uint16_t wmap[FAN ? (1 << PB) : 1];
+  __shared__ uint64_t fsum[FAN ? BT / 64 : 1];
   const uint32_t D = 1u << digit_bits;
   const uint64_t dmask = D - 1;
   const uint64_t total = (uint64_t)nseg * chunks;
+  uint32_t overflow = 0;
+  uint64_t fan_total = 0;
   for (uint64_t gi = blockIdx.x; gi < total; gi += gridDim.x) {
     const uint32_t s = (uint32_t)(gi / chunks), p = (uint32_t)(gi % chunks);
     uint64_t a = 0, b = 0;
@@ -930,10 +995,19 @@ __global__ __launch_bounds__(kPartBlock, 1) void k_part_scatter(const Rec<L>* __
     if constexpr (RUNS)
       nr = load_runs(rn, s, p, chunks, rt[0]);
     else
-      chunk_range(segs, s, p, chunks, &a, &b);
+      chunk_range(segs, in_cap, s, p, chunks, &a, &b);
     for (uint32_t d = threadIdx.x; d < D; d += BT) {
       st.count[d] = 0;
-      st.gpos[d] = offsets[((uint64_t)s * D + d) * chunks + p];
+      if constexpr (!FIXED) st.gpos[d] = offsets[((uint64_t)s * D + d) * chunks + p];
+    }
+    uint32_t w0 = 0, span = 0;
+    if constexpr (FAN) {
+      // merge workgroup of each digit's final bucket, relative to the first
+      w0 = (uint32_t)(((uint64_t)s * D) / fan.per);
+      span = (uint32_t)(((uint64_t)s * D + D - 1) / fan.per) - w0 + 1;
+      for (uint32_t i = threadIdx.x; i < span; i += BT) fl[i] = 0;
+      for (uint32_t d = threadIdx.x; d < D; d += BT)
+        wmap[d] = (uint16_t)((uint32_t)(((uint64_t)s * D + d) / fan.per) - w0);
     }
     lds_barrier();
     // Records [a, b): record indices, or (run mode) indices into the
@@ -965,17 +1039,40 @@ __global__ __launch_bounds__(kPartBlock, 1) void k_part_scatter(const Rec<L>* __
           if (i < b) nx[j] = load_stream(in + record_at(i));
         }
       }
-      // 1. digit and rank of every staged record
+      // 1. digit and rank of every staged record (and its fanout)
       uint32_t dg[PT], rk[PT];
 #pragma unroll
       for (int j = 0; j < PT; ++j) {
         const uint32_t slot = (uint32_t)j * BT + threadIdx.x;
-        dg[j] = (uint32_t)((mix64(rec_key(cur[j], kb)) >> shift) & dmask);
+        const uint64_t key = rec_key(cur[j], kb);
+        dg[j] = (uint32_t)((mix64(key) >> shift) & dmask);
         rk[j] = 0;
-        if (slot < nrec) rk[j] = atomicAdd(&st.count[dg[j]], 1u);
+        if (slot < nrec) {
+          rk[j] = atomicAdd(&st.count[dg[j]], 1u);
+          if constexpr (FAN) {
+            const uint32_t f = (uint32_t)__popc(legal_positions(key, fan.g));
+            if (f) {
+              atomicAdd(&fl[wmap[dg[j]]], f);
+              fan_total += f;
+            }
+          }
+        }
       }
       lds_barrier();
       scan_digits<BT>(st.count, st.start, D, st.wave_tot);
+      // FIXED: claim every digit's run of this stage in its output bucket
+      // (the claims' round trips overlap the placement below)
+      constexpr int kClaims = FIXED ? ((1 << PB) + BT - 1) / BT : 1;
+      uint64_t claim[kClaims];
+      if constexpr (FIXED) {
+#pragma unroll
+        for (int c = 0; c < kClaims; ++c) {
+          const uint32_t d = threadIdx.x + (uint32_t)c * BT;
+          claim[c] = 0;
+          if (d < D && st.count[d])
+            claim[c] = atomicAdd((unsigned long long*)&offsets[(uint64_t)s * D + d], (unsigned long long)st.count[d]);
+        }
+      }
       // 2. place the records sorted by digit
 #pragma unroll
       for (int j = 0; j < PT; ++j) {
@@ -986,27 +1083,48 @@ __global__ __launch_bounds__(kPartBlock, 1) void k_part_scatter(const Rec<L>* __
           st.digit[at] = (std::remove_reference_t<decltype(st.digit[0])>)dg[j];
         }
       }
+      if constexpr (FIXED) {
+#pragma unroll
+        for (int c = 0; c < kClaims; ++c) {
+          const uint32_t d = threadIdx.x + (uint32_t)c * BT;
+          if (d < D) st.gpos[d] = ((uint64_t)s * D + d) * out_cap + claim[c];
+        }
+      }
       lds_barrier();
       // 3. linear write-out of the digit runs
       for (uint32_t i = threadIdx.x; i < nrec; i += BT) {
         const uint32_t d = st.digit[i];
         const uint64_t pos = st.gpos[d] + (i - st.start[d]);
-        if (pos < cap)
+        bool ok = pos < cap;
+        if constexpr (FIXED) ok = ok && pos < ((uint64_t)s * D + d + 1) * out_cap;
+        if (ok)
           out[pos] = st.rec[i];
         else
-          atomicOr((unsigned long long*)&counters[kError], 2ull);
+          overflow |= FIXED && pos < cap ? 32u : 2u;
       }
       lds_barrier();
       for (uint32_t d = threadIdx.x; d < D; d += BT) {
-        st.gpos[d] += st.count[d];
+        if constexpr (!FIXED) st.gpos[d] += st.count[d];
         st.count[d] = 0;
       }
       lds_barrier();
     }
-    // The run table is rewritten by the next chunk: every wave must be past
-    // its reads (an empty chunk runs no stage and so no barrier above).
-    if constexpr (RUNS) lds_barrier();
+    if constexpr (FAN) {
+      lds_barrier();
+      for (uint32_t i = threadIdx.x; i < span; i += BT)
+        if (fl[i]) atomicAdd((unsigned long long*)&fan.U[w0 + i], (unsigned long long)fl[i]);
+    }
+    // The run table (and the fanout counters) are rewritten by the next
+    // chunk: every wave must be past its reads (an empty chunk runs no stage
+    // and so no barrier above).
+    if constexpr (RUNS || FAN) lds_barrier();
   }
+  if constexpr (FAN) {
+    const uint64_t t = block_sum_u64<BT / 64>(fan_total, fsum);
+    if (threadIdx.x == 0 && t) atomicAdd((unsigned long long*)fan.total, (unsigned long long)t);
+  }
+  if (overflow & 2u) atomicOr((unsigned long long*)&counters[kError], 2ull);
+  if (overflow & 32u) atomicOr((unsigned long long*)&counters[kRetry], 32ull);
 }
 
 // ----------------------------------------------------------- merge-expand --
@@ -1030,10 +1148,10 @@ __global__ __launch_bounds__(kPartBlock, 1) void k_part_scatter(const Rec<L>* __
 // sized 2^DB; SCATTER at <= 8 bits keeps four workgroups per CU).
 template <int L, int MODE, int T, int DB = kFineBits, int BT = kBlock>
 __global__ __launch_bounds__(BT, (BT == kScatterBlock && L <= 2 && T == 2048 && DB <= kNarrowDigitBits ? 8 : 1)) void k_merge_expand(
-    const Rec<L>* __restrict__ in, const uint64_t* __restrict__ segs, uint32_t nseg, uint32_t per,
-    RowGeom g, Primes pr, uint32_t digit_bits, uint32_t* __restrict__ hist,
-    const uint64_t* __restrict__ offsets, Rec<L>* __restrict__ out, uint64_t cap,
-    uint64_t final_mask, uint64_t* __restrict__ slice_sums, uint64_t nslices,
+    const Rec<L>* __restrict__ in, const uint64_t* __restrict__ segs, uint64_t in_cap, uint32_t nseg,
+    uint32_t per, RowGeom g, Primes pr, uint32_t digit_bits, uint32_t* __restrict__ hist,
+    const uint64_t* __restrict__ offsets, const uint64_t* __restrict__ capw, Rec<L>* __restrict__ out,
+    uint64_t cap, uint64_t final_mask, uint64_t* __restrict__ slice_sums, uint64_t nslices,
     uint64_t* __restrict__ counters) {
   constexpr uint32_t kCap = (T * 5) / 8;             // records per pass
   constexpr int KB = (int)((kCap + BT - 1) / BT);  // records per thread per batch
@@ -1105,20 +1223,20 @@ __global__ __launch_bounds__(BT, (BT == kScatterBlock && L <= 2 && T == 2048 && 
   };
   uint64_t plo = 0, phi = 0;
   if (b0 < b1) {
-    plo = segs[b0];
-    phi = segs[b0 + 1];
+    bucket_range(segs, in_cap, b0, &plo, &phi);
     if (phi - plo <= kCap) load_batch(plo, phi);
   }
+  // Capped SCATTER runs: at most wcap successors per digit (more sets bit 4
+  // of counters[kRetry] and writes nothing past the run; the engine then
+  // repeats the SCATTER with upper-bound offsets).
+  const uint64_t wcap = (MODE == kMergeScatter && capw) ? capw[blockIdx.x] : ~0ull;
 
   for (uint32_t b = b0; b < b1; ++b) {
     const uint64_t lo = plo, hi = phi;
     const uint64_t m = hi - lo;
     const uint32_t passes = m <= kCap ? 1u : (uint32_t)((m + kCap - 1) / kCap);
     uint64_t nlo = 0, nhi = 0;
-    if (b + 1 < b1) {
-      nlo = segs[b + 1];
-      nhi = segs[b + 2];
-    }
+    if (b + 1 < b1) bucket_range(segs, in_cap, b + 1, &nlo, &nhi);
     for (uint32_t pass = 0; pass < passes; ++pass) {
       uint32_t* const nu_ctr = &nuniq[parity];
       for (uint64_t base = lo; base < hi; base += kSpan) {
@@ -1303,7 +1421,9 @@ __global__ __launch_bounds__(BT, (BT == kScatterBlock && L <= 2 && T == 2048 && 
                 for (int l = 0; l < L; ++l) rec.r[l] = v[l];
               }
               const uint64_t pos = gpos[d] + rank;
-              if (pos < cap)
+              if (rank >= wcap)
+                overflow |= 16u;
+              else if (pos < cap)
                 out[pos] = rec;
               else
                 overflow |= 2u;
@@ -1364,7 +1484,8 @@ __global__ __launch_bounds__(BT, (BT == kScatterBlock && L <= 2 && T == 2048 && 
         if (ssl[i]) atomicAdd((unsigned long long*)&slice_sums[i], (unsigned long long)ssl[i]);
     }
   }
-  if (overflow) atomicOr((unsigned long long*)&counters[kError], (unsigned long long)overflow);
+  if (overflow & ~16u) atomicOr((unsigned long long*)&counters[kError], (unsigned long long)(overflow & ~16u));
+  if (overflow & 16u) atomicOr((unsigned long long*)&counters[kRetry], 16ull);
 }
 
 uint32_t grid_for(uint64_t items, uint32_t per_block, uint32_t cap) {
@@ -1446,43 +1567,35 @@ uint32_t partition_chunks(uint64_t records, uint64_t nseg) {
   return (uint32_t)(c < 1 ? 1 : (c > kMaxPartChunks ? kMaxPartChunks : c));
 }
 
-hipError_t launch_part_hist(int lanes, const void* in, const uint64_t* segs, uint32_t nseg, uint32_t chunks,
-                            uint32_t shift, uint32_t digit_bits, uint32_t key_bits, const Runs* runs,
-                            uint32_t* hist, const UbCount* ub, hipStream_t stream) {
+hipError_t launch_part_hist(int lanes, const void* in, const Buckets& bk, uint32_t chunks, uint32_t shift,
+                            uint32_t digit_bits, uint32_t key_bits, const Runs* runs, uint32_t* hist,
+                            hipStream_t stream) {
+  const uint32_t nseg = bk.nseg;
   if (digit_bits == 0 || digit_bits > kMaxDigitBits || nseg == 0 || chunks == 0) return hipErrorInvalidValue;
   if (((uint64_t)nseg << digit_bits) > 0xffffffffull) return hipErrorInvalidValue;
   if (runs && (runs->per_seg == 0 || (runs->per_seg + chunks - 1) / chunks > kMaxChunkRuns))
-    return hipErrorInvalidValue;
-  if (ub && (runs || ub->hist == nullptr || ub->W == 0 || !ub_fits(digit_bits, ub->per, ub->bits) ||
-             (uint64_t)ub->W * ub->per < ((uint64_t)nseg << digit_bits)))
     return hipErrorInvalidValue;
   const uint64_t total = (uint64_t)nseg * chunks;
   dim3 grid(total < 65536 ? (uint32_t)total : 65536u), block(kBlock);
   const bool narrow = digit_bits <= (uint32_t)kNarrowDigitBits;
   const bool mid = digit_bits <= (uint32_t)kMidDigitBits;
   const Runs rn = runs ? *runs : Runs{nullptr, nullptr, 0};
-  const UbCount uc = ub ? *ub : UbCount{nullptr, 0, 1, 0, RowGeom{}};
-#define PEPS_PH_ONE(LN, PB, RUNS, UB)                                                                     \
-  hipLaunchKernelGGL((k_part_hist<LN, PB, RUNS, UB>), grid, block, 0, stream, (const Rec<LN>*)in, segs, nseg, \
-                     chunks, shift, digit_bits, key_bits, rn, hist, uc)
-#define PEPS_PH_UB(LN, PB)                \
-  if (ub)                                 \
-    PEPS_PH_ONE(LN, PB, false, true);     \
-  else                                    \
-    PEPS_PH_ONE(LN, PB, false, false)
-#define PEPS_PH(LN)                                     \
-  if (narrow && runs)                                   \
-    PEPS_PH_ONE(LN, kNarrowDigitBits, true, false);     \
-  else if (narrow)                                      \
-    PEPS_PH_UB(LN, kNarrowDigitBits);                   \
-  else if (mid && runs)                                 \
-    PEPS_PH_ONE(LN, kMidDigitBits, true, false);        \
-  else if (mid)                                         \
-    PEPS_PH_UB(LN, kMidDigitBits);                      \
-  else if (runs)                                        \
-    PEPS_PH_ONE(LN, kMaxDigitBits, true, false);        \
-  else                                                  \
-    PEPS_PH_UB(LN, kMaxDigitBits);                      \
+#define PEPS_PH_ONE(LN, PB, RUNS)                                                                          \
+  hipLaunchKernelGGL((k_part_hist<LN, PB, RUNS>), grid, block, 0, stream, (const Rec<LN>*)in, bk.segs, bk.cap, \
+                     nseg, chunks, shift, digit_bits, key_bits, rn, hist)
+#define PEPS_PH(LN)                                                                      \
+  if (narrow && runs)                                                                    \
+    PEPS_PH_ONE(LN, kNarrowDigitBits, true);                                             \
+  else if (narrow)                                                                       \
+    PEPS_PH_ONE(LN, kNarrowDigitBits, false);                                            \
+  else if (mid && runs)                                                                  \
+    PEPS_PH_ONE(LN, kMidDigitBits, true);                                                \
+  else if (mid)                                                                          \
+    PEPS_PH_ONE(LN, kMidDigitBits, false);                                               \
+  else if (runs)                                                                         \
+    PEPS_PH_ONE(LN, kMaxDigitBits, true);                                                \
+  else                                                                                   \
+    PEPS_PH_ONE(LN, kMaxDigitBits, false);                                               \
   break
   switch (lanes) {
     case kPackedRecords: PEPS_PH(kPackedRecords);
@@ -1493,42 +1606,55 @@ hipError_t launch_part_hist(int lanes, const void* in, const uint64_t* segs, uin
     default: return hipErrorInvalidValue;
   }
 #undef PEPS_PH
-#undef PEPS_PH_UB
 #undef PEPS_PH_ONE
   return hipGetLastError();
 }
 
-hipError_t launch_part_scatter(int lanes, const void* in, void* out, const uint64_t* segs, uint32_t nseg,
-                               uint32_t chunks, uint32_t shift, uint32_t digit_bits, uint32_t key_bits,
-                               const Runs* runs, const uint64_t* offsets, uint64_t cap, uint64_t* counters,
-                               hipStream_t stream) {
+hipError_t launch_part_scatter(int lanes, const void* in, void* out, const Buckets& bk, uint32_t chunks,
+                               uint32_t shift, uint32_t digit_bits, uint32_t key_bits, const Runs* runs,
+                               uint64_t* offsets, uint64_t out_cap, uint64_t cap, uint64_t* counters,
+                               const FanSum* fan, hipStream_t stream) {
+  const uint32_t nseg = bk.nseg;
   if (digit_bits == 0 || digit_bits > kMaxDigitBits || nseg == 0 || chunks == 0) return hipErrorInvalidValue;
   if (runs && (runs->per_seg == 0 || (runs->per_seg + chunks - 1) / chunks > kMaxChunkRuns))
+    return hipErrorInvalidValue;
+  if (out_cap && ((uint64_t)nseg << digit_bits) > cap / out_cap) return hipErrorInvalidValue;
+  if (fan && (out_cap == 0 || fan->U == nullptr || fan->total == nullptr || fan->per == 0 ||
+              (uint64_t)fan->W * fan->per < ((uint64_t)nseg << digit_bits)))
     return hipErrorInvalidValue;
   const uint64_t total = (uint64_t)nseg * chunks;
   dim3 grid(total < 65536 ? (uint32_t)total : 65536u);
   const bool narrow = digit_bits <= (uint32_t)kNarrowDigitBits;
   const bool mid = digit_bits <= (uint32_t)kMidDigitBits;
   const Runs rn = runs ? *runs : Runs{nullptr, nullptr, 0};
-#define PEPS_PS(LN)                                                                                            \
-  if (narrow && runs)                                                                                          \
-    hipLaunchKernelGGL((k_part_scatter<LN, kNarrowDigitBits, true>), grid, dim3(kPartBlock), 0, stream, (const Rec<LN>*)in,\
-                       (Rec<LN>*)out, segs, nseg, chunks, shift, digit_bits, key_bits, rn, offsets, cap, counters); \
-  else if (narrow)                                                                                             \
-    hipLaunchKernelGGL((k_part_scatter<LN, kNarrowDigitBits, false>), grid, dim3(kPartBlock), 0, stream, (const Rec<LN>*)in,\
-                       (Rec<LN>*)out, segs, nseg, chunks, shift, digit_bits, key_bits, rn, offsets, cap, counters); \
-  else if (mid && runs)                                                                                        \
-    hipLaunchKernelGGL((k_part_scatter<LN, kMidDigitBits, true>), grid, dim3(kPartBlock), 0, stream, (const Rec<LN>*)in,\
-                       (Rec<LN>*)out, segs, nseg, chunks, shift, digit_bits, key_bits, rn, offsets, cap, counters); \
-  else if (mid)                                                                                                \
-    hipLaunchKernelGGL((k_part_scatter<LN, kMidDigitBits, false>), grid, dim3(kPartBlock), 0, stream, (const Rec<LN>*)in,\
-                       (Rec<LN>*)out, segs, nseg, chunks, shift, digit_bits, key_bits, rn, offsets, cap, counters); \
-  else if (runs)                                                                                               \
-    hipLaunchKernelGGL((k_part_scatter<LN, kMaxDigitBits, true>), grid, dim3(kPartBlock), 0, stream, (const Rec<LN>*)in,  \
-                       (Rec<LN>*)out, segs, nseg, chunks, shift, digit_bits, key_bits, rn, offsets, cap, counters); \
-  else                                                                                                         \
-    hipLaunchKernelGGL((k_part_scatter<LN, kMaxDigitBits, false>), grid, dim3(kPartBlock), 0, stream, (const Rec<LN>*)in, \
-                       (Rec<LN>*)out, segs, nseg, chunks, shift, digit_bits, key_bits, rn, offsets, cap, counters); \
+  const FanSum fs = fan ? *fan : FanSum{nullptr, nullptr, 0, 1, RowGeom{}};
+  const bool fixed = out_cap != 0;
+#define PEPS_PS_ONE(LN, PB, RUNS, FIXED, FAN)                                                                    \
+  hipLaunchKernelGGL((k_part_scatter<LN, PB, RUNS, FIXED, FAN>), grid, dim3(kPartBlock), 0, stream,                \
+                     (const Rec<LN>*)in, (Rec<LN>*)out, bk.segs, bk.cap, nseg, chunks, shift, digit_bits, key_bits, \
+                     rn, offsets, out_cap, cap, counters, fs)
+  // Run-mode levels read SCATTER's runs; fixed-capacity levels may fold in
+  // the fanout sums; exact levels are the fallback and the materialised path.
+#define PEPS_PS_PB(LN, PB)                            \
+  if (runs && fixed && fan)                           \
+    PEPS_PS_ONE(LN, PB, true, true, true);            \
+  else if (runs && fixed)                             \
+    PEPS_PS_ONE(LN, PB, true, true, false);           \
+  else if (runs)                                      \
+    PEPS_PS_ONE(LN, PB, true, false, false);          \
+  else if (fixed && fan)                              \
+    PEPS_PS_ONE(LN, PB, false, true, true);           \
+  else if (fixed)                                     \
+    PEPS_PS_ONE(LN, PB, false, true, false);          \
+  else                                                \
+    PEPS_PS_ONE(LN, PB, false, false, false)
+#define PEPS_PS(LN)                                   \
+  if (narrow)                                         \
+    PEPS_PS_PB(LN, kNarrowDigitBits);                 \
+  else if (mid)                                       \
+    PEPS_PS_PB(LN, kMidDigitBits);                    \
+  else                                                \
+    PEPS_PS_PB(LN, kMaxDigitBits);                    \
   break
   switch (lanes) {
     case kPackedRecords: PEPS_PS(kPackedRecords);
@@ -1539,21 +1665,41 @@ hipError_t launch_part_scatter(int lanes, const void* in, void* out, const uint6
     default: return hipErrorInvalidValue;
   }
 #undef PEPS_PS
+#undef PEPS_PS_PB
+#undef PEPS_PS_ONE
   return hipGetLastError();
 }
 
-hipError_t launch_count_ub(int lanes, const void* in, const uint64_t* segs, uint32_t nseg, uint32_t W,
-                           const RowGeom& g, uint32_t digit_bits, uint32_t* hist, hipStream_t stream) {
+hipError_t launch_count_ub(int lanes, const void* in, const Buckets& bk, uint32_t W, const RowGeom& g,
+                           uint32_t digit_bits, uint32_t* hist, hipStream_t stream) {
+  const uint32_t nseg = bk.nseg;
   if (nseg == 0 || W == 0 || digit_bits > kFineBits) return hipErrorInvalidValue;
   const uint32_t per = (nseg + W - 1) / W;
+#define PEPS_CU(LN)                                                                                              \
+  hipLaunchKernelGGL(k_count_ub<LN>, dim3(W), dim3(kBlock), 0, stream, (const Rec<LN>*)in, bk.segs, bk.cap, nseg, \
+                     per, g, digit_bits, hist)
   switch (lanes) {
-    case kPackedRecords: hipLaunchKernelGGL(k_count_ub<kPackedRecords>, dim3(W), dim3(kBlock), 0, stream, (const Packed*)in, segs, nseg, per, g, digit_bits, hist); break;
-    case 1: hipLaunchKernelGGL(k_count_ub<1>, dim3(W), dim3(kBlock), 0, stream, (const Rec<1>*)in, segs, nseg, per, g, digit_bits, hist); break;
-    case 2: hipLaunchKernelGGL(k_count_ub<2>, dim3(W), dim3(kBlock), 0, stream, (const Rec<2>*)in, segs, nseg, per, g, digit_bits, hist); break;
-    case 3: hipLaunchKernelGGL(k_count_ub<3>, dim3(W), dim3(kBlock), 0, stream, (const Rec<3>*)in, segs, nseg, per, g, digit_bits, hist); break;
-    case 4: hipLaunchKernelGGL(k_count_ub<4>, dim3(W), dim3(kBlock), 0, stream, (const Rec<4>*)in, segs, nseg, per, g, digit_bits, hist); break;
+    case kPackedRecords: PEPS_CU(kPackedRecords); break;
+    case 1: PEPS_CU(1); break;
+    case 2: PEPS_CU(2); break;
+    case 3: PEPS_CU(3); break;
+    case 4: PEPS_CU(4); break;
     default: return hipErrorInvalidValue;
   }
+#undef PEPS_CU
+  return hipGetLastError();
+}
+
+hipError_t launch_caps(const uint64_t* U, uint32_t W, uint32_t digit_bits, uint32_t shrink, uint64_t* capw,
+                       uint64_t* pre, uint64_t* run_start, uint64_t* counters, hipStream_t stream) {
+  if (W == 0 || digit_bits == 0 || digit_bits > kFineBits || shrink > 16) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_caps_scan, dim3(1), dim3(kCapsBlock), 0, stream, U, W, digit_bits, shrink, capw, pre,
+                     counters);
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  const uint64_t total = (uint64_t)W << digit_bits;
+  hipLaunchKernelGGL(k_caps_expand, dim3(grid_for(total, kBlock * 4, 4096)), dim3(kBlock), 0, stream, pre, W,
+                     digit_bits, run_start);
   return hipGetLastError();
 }
 
@@ -1593,31 +1739,31 @@ hipError_t launch_site_scatter(int lanes, const uint64_t* keys, const uint32_t* 
 
 template <int L, int T>
 static hipError_t launch_me(int mode, dim3 grid, hipStream_t stream, const void* in,
-                            const uint64_t* segs, uint32_t nseg, uint32_t per, const RowGeom& g,
+                            const uint64_t* segs, uint64_t in_cap, uint32_t nseg, uint32_t per, const RowGeom& g,
                             const Primes& pr, uint32_t digit_bits, uint32_t* hist,
-                            const uint64_t* offsets, void* out, uint64_t cap, uint64_t final_mask,
-                            uint64_t* slice_sums, uint64_t nslices, uint64_t* counters) {
+                            const uint64_t* offsets, const uint64_t* capw, void* out, uint64_t cap,
+                            uint64_t final_mask, uint64_t* slice_sums, uint64_t nslices, uint64_t* counters) {
   const Rec<L>* i = (const Rec<L>*)in;
   Rec<L>* o = (Rec<L>*)out;
   switch (mode) {
     case kMergeCount:
       if (digit_bits <= kNarrowDigitBits)
-        hipLaunchKernelGGL((k_merge_expand<L, kMergeCount, T, kNarrowDigitBits>), grid, dim3(kBlock), 0, stream, i, segs, nseg, per, g, pr, digit_bits, hist, offsets, o, cap, final_mask, slice_sums, nslices, counters);
+        hipLaunchKernelGGL((k_merge_expand<L, kMergeCount, T, kNarrowDigitBits>), grid, dim3(kBlock), 0, stream, i, segs, in_cap, nseg, per, g, pr, digit_bits, hist, offsets, capw, o, cap, final_mask, slice_sums, nslices, counters);
       else
-        hipLaunchKernelGGL((k_merge_expand<L, kMergeCount, T>), grid, dim3(kBlock), 0, stream, i, segs, nseg, per, g, pr, digit_bits, hist, offsets, o, cap, final_mask, slice_sums, nslices, counters);
+        hipLaunchKernelGGL((k_merge_expand<L, kMergeCount, T>), grid, dim3(kBlock), 0, stream, i, segs, in_cap, nseg, per, g, pr, digit_bits, hist, offsets, capw, o, cap, final_mask, slice_sums, nslices, counters);
       break;
     case kMergeScatter:
       if (digit_bits <= kNarrowDigitBits)
-        hipLaunchKernelGGL((k_merge_expand<L, kMergeScatter, T, kNarrowDigitBits, kScatterBlock>), grid, dim3(kScatterBlock), 0, stream, i, segs, nseg, per, g, pr, digit_bits, hist, offsets, o, cap, final_mask, slice_sums, nslices, counters);
+        hipLaunchKernelGGL((k_merge_expand<L, kMergeScatter, T, kNarrowDigitBits, kScatterBlock>), grid, dim3(kScatterBlock), 0, stream, i, segs, in_cap, nseg, per, g, pr, digit_bits, hist, offsets, capw, o, cap, final_mask, slice_sums, nslices, counters);
       else
-        hipLaunchKernelGGL((k_merge_expand<L, kMergeScatter, T, kFineBits, kScatterBlock>), grid, dim3(kScatterBlock), 0, stream, i, segs, nseg, per, g, pr, digit_bits, hist, offsets, o, cap, final_mask, slice_sums, nslices, counters);
+        hipLaunchKernelGGL((k_merge_expand<L, kMergeScatter, T, kFineBits, kScatterBlock>), grid, dim3(kScatterBlock), 0, stream, i, segs, in_cap, nseg, per, g, pr, digit_bits, hist, offsets, capw, o, cap, final_mask, slice_sums, nslices, counters);
       break;
-    case kMergeLast: hipLaunchKernelGGL((k_merge_expand<L, kMergeLast, T>), grid, dim3(kBlock), 0, stream, i, segs, nseg, per, g, pr, digit_bits, hist, offsets, o, cap, final_mask, slice_sums, nslices, counters); break;
+    case kMergeLast: hipLaunchKernelGGL((k_merge_expand<L, kMergeLast, T>), grid, dim3(kBlock), 0, stream, i, segs, in_cap, nseg, per, g, pr, digit_bits, hist, offsets, capw, o, cap, final_mask, slice_sums, nslices, counters); break;
     case kMergeEmit:
       if constexpr (L == kPackedRecords)
         return hipErrorInvalidValue;
       else
-        hipLaunchKernelGGL((k_merge_expand<L, kMergeEmit, T>), grid, dim3(kBlock), 0, stream, i, segs, nseg, per, g, pr, digit_bits, hist, offsets, o, cap, final_mask, slice_sums, nslices, counters);
+        hipLaunchKernelGGL((k_merge_expand<L, kMergeEmit, T>), grid, dim3(kBlock), 0, stream, i, segs, in_cap, nseg, per, g, pr, digit_bits, hist, offsets, capw, o, cap, final_mask, slice_sums, nslices, counters);
       break;
     default: return hipErrorInvalidValue;
   }
@@ -1654,16 +1800,16 @@ uint32_t merge_resident_blocks(int lanes, uint32_t table_slots, uint32_t digit_b
   return (uint32_t)blocks;
 }
 
-hipError_t launch_merge_expand(int mode, int lanes, const void* in, const uint64_t* segs,
-                               uint32_t nseg, uint32_t W, const RowGeom& g, const Primes& pr,
-                               uint32_t digit_bits, uint32_t* hist, const uint64_t* offsets,
-                               void* out, uint64_t cap, uint64_t final_mask, uint64_t* slice_sums,
-                               uint64_t nslices, uint64_t* counters, uint32_t table_slots,
-                               hipStream_t stream) {
+hipError_t launch_merge_expand(int mode, int lanes, const void* in, const Buckets& bk, uint32_t W,
+                               const RowGeom& g, const Primes& pr, uint32_t digit_bits, uint32_t* hist,
+                               const uint64_t* offsets, const uint64_t* capw, void* out, uint64_t cap,
+                               uint64_t final_mask, uint64_t* slice_sums, uint64_t nslices, uint64_t* counters,
+                               uint32_t table_slots, hipStream_t stream) {
+  const uint32_t nseg = bk.nseg;
   if (nseg == 0 || W == 0 || digit_bits > kFineBits) return hipErrorInvalidValue;
   const uint32_t per = (nseg + W - 1) / W;
   dim3 grid(W);
-#define PEPS_ME(LN, TS) return launch_me<LN, TS>(mode, grid, stream, in, segs, nseg, per, g, pr, digit_bits, hist, offsets, out, cap, final_mask, slice_sums, nslices, counters)
+#define PEPS_ME(LN, TS) return launch_me<LN, TS>(mode, grid, stream, in, bk.segs, bk.cap, nseg, per, g, pr, digit_bits, hist, offsets, capw, out, cap, final_mask, slice_sums, nslices, counters)
   if (table_slots == 2048) {
     switch (lanes) {
       case kPackedRecords: PEPS_ME(kPackedRecords, 2048);

@@ -145,7 +145,7 @@ struct peps_server {
       std::string e;
       peps::Engine* eng = peps::Engine::get((int)res.device, &e);
       const double ratio = projection.ratio(n, depth);
-      const double cap = eng ? 0.8 * (double)eng->candidate_capacity((int)res.primes.size(), res.records) : 0.0;
+      const double cap = eng ? peps::kAdmitFraction * (double)eng->candidate_capacity((int)res.primes.size(), res.records) : 0.0;
       const size_t slices = peps::max_slices(n, res.slice_group);
       const size_t limit = ratio > 0 ? slices : std::min<size_t>(slices, 8);
       std::vector<peps::Task> batch;
@@ -216,7 +216,7 @@ struct peps_server {
           double psum = 0;
           for (size_t i = a; i < b; ++i) psum += bproxy[i];
           std::lock_guard<std::mutex> g(mu);
-          projection.exceeded(n, depth, psum, cap / 0.8);
+          projection.exceeded(n, depth, psum, cap / peps::kAdmitFraction);
           ++retries;
         };
         rc = peps::split_retry(0, batch.size(), run, exceeded);

@@ -33,6 +33,10 @@ PEPS_SYMMETRY_TOP_ROW_ORBITS = 1
 PEPS_RECORDS_AUTO = 0
 PEPS_RECORDS_LANES = 1
 PEPS_RECORDS_PACKED = 2
+PEPS_FORCE_EXACT_LEVELS = 1
+PEPS_FORCE_UPPER_RUNS = 2
+PEPS_FORCE_LEVEL_OVERFLOW = 4
+PEPS_FORCE_CAP_OVERFLOW = 8
 
 
 # peps_progress_fn: (user, done, total)

@@ -64,6 +64,7 @@ class PepsOptions:
     site_counters: int = 0      # sitewise tensor counters: 0 indexed, 1 linear scan of C
     record_format: int = 0      # PEPS_RECORDS_*: 0 packed (lane fallback), 1 lane records, 2 packed only
     packed_weight_bits: int = 0  # nonzero: narrower packed weight field (exercises the fallback)
+    force_paths: int = 0         # FORCE_* bits: take the pipeline's fallback paths (tests; same results)
 
     def to_c(self) -> Options:
         o = Options()
@@ -79,7 +80,7 @@ class PepsOptions:
         o.merge_passes = self.merge_passes
         o.merge_waves = self.merge_waves
         o.site_counters = self.site_counters
-        o.record_format = self.record_format | (self.packed_weight_bits << 8)
+        o.record_format = self.record_format | (self.packed_weight_bits << 8) | (self.force_paths << 16)
         return o
 
 

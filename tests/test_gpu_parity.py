@@ -135,38 +135,44 @@ def test_batched_n16_two_row_tasks():
     assert b.count.value == 14772512 and tasks == len(o.prefix_tasks(16, 2))
 
 
-def test_fused_and_separate_upper_bound_count_agree():
-    # Rows of 2^20+ candidates size their SCATTER from an upper-bound COUNT.
-    # With the default 6-bit SCATTER digit the large rows' count is folded
-    # into the last partition level's histogram launch; a 10-bit digit has
-    # no room in its LDS counters, so there it stays a pass of its own.
-    def run(opts):
-        pm.profile_kernels(True)
-        try:
-            r = pm.contract_rows(17, opts)
-            ks = pm.kernel_stats()
-        finally:
-            pm.profile_kernels(False)
-        return r, ks
-
-    fused, kf = run(None)
-    sep, ksep = run(pm.PepsOptions(expand_bits=10))
-    show("fused", sorted(kf), "separate", sorted(ksep))
-    assert any(k.startswith("part_hist") and k.endswith("_ub") for k in kf)
-    assert "count_ub" in ksep
-    assert not any(k.startswith("part_hist") and k.endswith("_ub") for k in ksep)
-    assert fused.count.value == sep.count.value == o.known_count(17)
-    assert tuples(fused) == tuples(sep)
-    # Graph replay keeps the fused count's zeroed histogram and its buffer
-    # (plain: call 1 records the size plan; batched: calls 3+ replay).
+@pytest.mark.parametrize("force", [0, 1, 2, 4, 8, 3])
+def test_fallback_paths_agree_layer_by_layer(force):
+    # The partition writes fixed-capacity buckets by claims and large rows'
+    # SCATTER writes runs capped from fanout sums; the force bits make the
+    # engine take the exact histogram levels (1), upper-bound-count runs (2),
+    # or capacities too small so that every level (4) or capped SCATTER (8)
+    # overflows and is repeated by its fallback.  Every path gives the same
+    # layers as the oracle's census at N = 17.
+    from peps_mi355x.abi import (PEPS_FORCE_CAP_OVERFLOW, PEPS_FORCE_EXACT_LEVELS, PEPS_FORCE_LEVEL_OVERFLOW,
+                                 PEPS_FORCE_UPPER_RUNS)
+    want = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "orbit_layers_n17_n18.json")))
+    want = [tuple(l[k] for k in KEYS) for l in want["boards"]["17"]["layers"]]
+    pm.profile_kernels(True)
+    try:
+        r = pm.contract_rows(17, pm.PepsOptions(force_paths=force))
+        ks = pm.kernel_stats()
+    finally:
+        pm.profile_kernels(False)
+    show("force", force, sorted(ks))
+    assert tuples(r) == want and r.count.value == o.known_count(17)
+    if force == 0:
+        assert "caps" in ks and "count_ub" not in ks
+    if force & (PEPS_FORCE_EXACT_LEVELS | PEPS_FORCE_LEVEL_OVERFLOW):
+        assert "part_hist" in ks or "part_hist_runs" in ks
+    if force & (PEPS_FORCE_UPPER_RUNS | PEPS_FORCE_CAP_OVERFLOW | PEPS_FORCE_EXACT_LEVELS |
+                PEPS_FORCE_LEVEL_OVERFLOW):
+        assert "count_ub" in ks
+    if force == PEPS_FORCE_CAP_OVERFLOW:
+        assert "caps" in ks
+    # Graph replay records the path the eager run settled on.
     for rep in range(3):
-        show("fused graph", rep)
-        g = pm.contract_rows(17, pm.PepsOptions(graph_mode=1))
-        assert tuples(g) == tuples(fused), rep
-    eager_b = pm.count_batched(17, 2, task_counts=True)
-    for rep in range(4):
-        b = pm.count_batched(17, 2, pm.PepsOptions(graph_mode=1), task_counts=True)
-        assert b[2] == eager_b[2] and b[0].count.value == o.known_count(17), rep
+        g = pm.contract_rows(17, pm.PepsOptions(force_paths=force, graph_mode=1))
+        assert tuples(g) == want, rep
+    if force in (0, 4):
+        eager_b = pm.count_batched(17, 2, pm.PepsOptions(force_paths=force), task_counts=True)
+        for rep in range(3):
+            b = pm.count_batched(17, 2, pm.PepsOptions(force_paths=force, graph_mode=1), task_counts=True)
+            assert b[2] == eager_b[2] and b[0].count.value == o.known_count(17), rep
 
 
 def test_repeated_runs_identical():

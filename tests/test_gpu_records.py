@@ -135,9 +135,9 @@ def test_published_crt_release_n18_n19():
 
 
 @pytest.mark.parametrize("lanes,record_bytes", [(2, 16), (3, 24)])
-def test_lane_records_through_the_fused_upper_bound_count(lanes, record_bytes):
-    # N = 17's large rows fold their upper-bound COUNT into the last
-    # partition level; lane records take their own instantiations of it.
+def test_lane_records_through_capped_runs_and_fixed_buckets(lanes, record_bytes):
+    # N = 17's large rows write capped SCATTER runs and fixed-capacity
+    # partition buckets; lane records take their own instantiations of them.
     p = pm.contract_rows(17, opts(PEPS_RECORDS_AUTO, crt_lanes=lanes))
     pm.profile_kernels(True)
     try:
@@ -147,7 +147,7 @@ def test_lane_records_through_the_fused_upper_bound_count(lanes, record_bytes):
         pm.profile_kernels(False)
     show("lanes", lanes, sorted(ks))
     assert (p.record_bytes, q.record_bytes) == (12, record_bytes)
-    assert any(k.startswith("part_hist") and k.endswith("_ub") for k in ks)
+    assert "caps" in ks
     assert p.count.value == q.count.value == o.known_count(17)
     assert list(q.residues)[:lanes] == [o.known_count(17) % x for x in list(q.primes)[:lanes]]
     assert tuples(p) == tuples(q)
