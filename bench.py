@@ -294,6 +294,8 @@ def main():
                     help="sweep the launch shape first (bounded by --autotune-seconds) and time the best")
     ap.add_argument("--autotune-seconds", type=float, default=240.0)
     ap.add_argument("--depth", type=int, default=2)
+    ap.add_argument("--options", default="",
+                    help='engine options as JSON, e.g. \'{"merge_passes": 2}\' (PepsOptions fields)')
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=30.0)
     ap.add_argument("--profile-steps", type=int, default=1,
@@ -327,6 +329,12 @@ def main():
                                   budget_s=args.autotune_seconds,
                                   progress=lambda sh, t: log(f"autotune {sh}: {t}"))
         log(f"autotune best: {opts}")
+    if args.options:
+        opts = opts or pm.PepsOptions()
+        for k, v in json.loads(args.options).items():
+            if not hasattr(opts, k):
+                raise SystemExit(f"unknown engine option {k}")
+            setattr(opts, k, v)
     if args.lanes or args.graph:
         opts = opts or pm.PepsOptions()
         opts.crt_lanes = args.lanes

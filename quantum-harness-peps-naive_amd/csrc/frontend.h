@@ -45,7 +45,7 @@ struct Resolved {
   std::vector<uint64_t> primes;
   uint32_t radix_bits = 8;
   uint32_t tile_states = 1024;
-  uint32_t expand_bits = 6;
+  uint32_t expand_bits = 5;
   uint32_t device = 0;
   uint64_t budget = 0;
   bool graph = false;

@@ -258,20 +258,21 @@ __device__ __forceinline__ uint64_t block_sum_u64(uint64_t v, uint64_t* scratch)
 
 // Exclusive scan of counts[0..D) in LDS into starts[0..D) by a workgroup of
 // BT threads.
-template <int BT = kBlock>
-__device__ __forceinline__ void scan_digits(const uint32_t* counts, uint32_t* starts, uint32_t D,
+// (Counts may be u64 words whose low half is the count.)
+template <int BT = kBlock, typename CT = uint32_t>
+__device__ __forceinline__ void scan_digits(const CT* counts, uint32_t* starts, uint32_t D,
                                             uint32_t* wave_tot) {
   const uint32_t per = (D + BT - 1) / BT;
   const uint32_t lo = threadIdx.x * per;
   uint32_t local = 0;
   for (uint32_t i = 0; i < per; ++i)
-    if (lo + i < D) local += counts[lo + i];
+    if (lo + i < D) local += (uint32_t)counts[lo + i];
   uint32_t total;
   uint32_t run = block_exclusive_scan<BT / 64>(local, &total, wave_tot);
   for (uint32_t i = 0; i < per; ++i) {
     if (lo + i < D) {
       starts[lo + i] = run;
-      run += counts[lo + i];
+      run += (uint32_t)counts[lo + i];
     }
   }
   lds_barrier();
@@ -940,7 +941,7 @@ constexpr int part_stage() {
   // gpos per digit, the run table and the wave totals, and (FAN) the
   // fanout counters of the merge workgroups one segment reaches
   constexpr int per = (int)sizeof(Rec<L>) + (PB <= 8 ? 1 : 2);
-  constexpr int fixed = (1 << PB) * 16 + 1024 + (FAN ? ((1 << PB) + 2) * 4 + (1 << PB) * 2 + kPartBlock / 8 : 0);
+  constexpr int fixed = (1 << PB) * 16 + 1024 + (FAN ? ((1 << PB) + 2) * 4 + (1 << PB) * 6 + kPartBlock / 8 : 0);
   return (kPartLds - fixed) / per / kPartBlock * kPartBlock;
 }
 
@@ -948,7 +949,9 @@ template <int L, int PB, bool FAN>
 struct PartStage {
   Rec<L> rec[part_stage<L, PB, FAN>()];
   std::conditional_t<(PB <= 8), uint8_t, uint16_t> digit[part_stage<L, PB, FAN>()];
-  uint32_t count[1 << PB];
+  // per digit: the stage's count (FAN: in the low word, the records' fanout
+  // sum in the high word, both taken by one 64-bit LDS atomic)
+  std::conditional_t<FAN, uint64_t, uint32_t> count[1 << PB];
   uint32_t start[1 << PB];
   uint64_t gpos[1 << PB];
   uint32_t wave_tot[kPartBlock / 64];
@@ -1048,18 +1051,24 @@ __global__ __launch_bounds__(kPartBlock, 1) void k_part_scatter(const Rec<L>* __
         dg[j] = (uint32_t)((mix64(key) >> shift) & dmask);
         rk[j] = 0;
         if (slot < nrec) {
-          rk[j] = atomicAdd(&st.count[dg[j]], 1u);
           if constexpr (FAN) {
             const uint32_t f = (uint32_t)__popc(legal_positions(key, fan.g));
-            if (f) {
-              atomicAdd(&fl[wmap[dg[j]]], f);
-              fan_total += f;
-            }
+            rk[j] = (uint32_t)atomicAdd((unsigned long long*)&st.count[dg[j]], ((unsigned long long)f << 32) | 1ull);
+            fan_total += f;
+          } else {
+            rk[j] = atomicAdd(&st.count[dg[j]], 1u);
           }
         }
       }
       lds_barrier();
       scan_digits<BT>(st.count, st.start, D, st.wave_tot);
+      // FAN: the stage's fanout per digit, onto its merge workgroup's counter
+      if constexpr (FAN) {
+        for (uint32_t d = threadIdx.x; d < D; d += BT) {
+          const uint32_t f = (uint32_t)(st.count[d] >> 32);
+          if (f) atomicAdd(&fl[wmap[d]], f);
+        }
+      }
       // FIXED: claim every digit's run of this stage in its output bucket
       // (the claims' round trips overlap the placement below)
       constexpr int kClaims = FIXED ? ((1 << PB) + BT - 1) / BT : 1;
@@ -1069,8 +1078,8 @@ __global__ __launch_bounds__(kPartBlock, 1) void k_part_scatter(const Rec<L>* __
         for (int c = 0; c < kClaims; ++c) {
           const uint32_t d = threadIdx.x + (uint32_t)c * BT;
           claim[c] = 0;
-          if (d < D && st.count[d])
-            claim[c] = atomicAdd((unsigned long long*)&offsets[(uint64_t)s * D + d], (unsigned long long)st.count[d]);
+          const uint32_t cnt = d < D ? (uint32_t)st.count[d] : 0u;
+          if (cnt) claim[c] = atomicAdd((unsigned long long*)&offsets[(uint64_t)s * D + d], (unsigned long long)cnt);
         }
       }
       // 2. place the records sorted by digit
@@ -1104,7 +1113,7 @@ __global__ __launch_bounds__(kPartBlock, 1) void k_part_scatter(const Rec<L>* __
       }
       lds_barrier();
       for (uint32_t d = threadIdx.x; d < D; d += BT) {
-        if constexpr (!FIXED) st.gpos[d] += st.count[d];
+        if constexpr (!FIXED) st.gpos[d] += (uint32_t)st.count[d];
         st.count[d] = 0;
       }
       lds_barrier();
@@ -1281,7 +1290,7 @@ __global__ __launch_bounds__(BT, (BT == kScatterBlock && L <= 2 && T == 2048 && 
           const int leader = __ffsll((long long)bal) - 1;
           uint32_t base_idx = 0;
           if (lane == leader) base_idx = atomicAdd(nu_ctr, (uint32_t)__popcll(bal));
-          base_idx = __shfl(base_idx, leader, 64);
+          base_idx = __builtin_amdgcn_readlane(base_idx, leader);
           if (o) {
             const uint32_t r = __builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32),
                                                          __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
@@ -1411,6 +1420,9 @@ __global__ __launch_bounds__(BT, (BT == kScatterBlock && L <= 2 && T == 2048 && 
             if (MODE == kMergeCount) {
               atomicAdd(&lcount[d], 1u);
             } else {
+              // The run's start is read before the rank is taken, so both
+              // LDS accesses are in flight together.
+              const uint64_t run0 = gpos[d];
               const uint32_t rank = atomicAdd(&lcount[d], 1u);
               Rec<L> rec;
               if constexpr (P) {
@@ -1420,13 +1432,12 @@ __global__ __launch_bounds__(BT, (BT == kScatterBlock && L <= 2 && T == 2048 && 
 #pragma unroll
                 for (int l = 0; l < L; ++l) rec.r[l] = v[l];
               }
-              const uint64_t pos = gpos[d] + rank;
-              if (rank >= wcap)
-                overflow |= 16u;
-              else if (pos < cap)
+              const uint64_t pos = run0 + rank;
+              const bool in_run = rank < wcap, in_region = pos < cap;
+              if (in_run && in_region)
                 out[pos] = rec;
               else
-                overflow |= 2u;
+                overflow |= in_run ? 2u : 16u;
             }
           }
         }
