@@ -33,6 +33,7 @@ constexpr int kWaves = kBlock / 64;
 constexpr int kStage = 2048;          // LDS-staged records per flush
 constexpr int kPerThread = kStage / kBlock;
 constexpr int kScatterBlock = 512;    // SCATTER merge workgroup: 8 wave64s share one LDS table
+constexpr int kXcds = 8;              // MI355X: accelerator complex dies, one L2 each
 
 template <int L>
 struct alignas(8) Rec {
@@ -991,8 +992,18 @@ __global__ __launch_bounds__(kPartBlock, 1) void k_part_scatter(const Rec<L>* __
   const uint64_t total = (uint64_t)nseg * chunks;
   uint32_t overflow = 0;
   uint64_t fan_total = 0;
-  for (uint64_t gi = blockIdx.x; gi < total; gi += gridDim.x) {
-    const uint32_t s = (uint32_t)(gi / chunks), p = (uint32_t)(gi % chunks);
+  // XCD-aware work order: workgroups are dispatched to the XCDs round robin,
+  // so workgroup b runs on XCD b % kXcds.  Every chunk of a segment goes to
+  // the same XCD, so the stages that append to one output bucket share an
+  // L2, which joins their partly written lines before they reach HBM.
+  const bool by_xcd = nseg >= (uint32_t)kXcds && gridDim.x % kXcds == 0;
+  const uint32_t xcd = blockIdx.x % kXcds;
+  const uint64_t items = by_xcd ? (uint64_t)((nseg - xcd + kXcds - 1) / kXcds) * chunks : total;
+  const uint64_t first = by_xcd ? blockIdx.x / kXcds : blockIdx.x;
+  const uint64_t stride = by_xcd ? gridDim.x / kXcds : gridDim.x;
+  for (uint64_t gi = first; gi < items; gi += stride) {
+    const uint32_t s = by_xcd ? xcd + (uint32_t)kXcds * (uint32_t)(gi / chunks) : (uint32_t)(gi / chunks);
+    const uint32_t p = (uint32_t)(gi % chunks);
     uint64_t a = 0, b = 0;
     uint32_t nr = 0;
     if constexpr (RUNS)
@@ -1634,7 +1645,10 @@ hipError_t launch_part_scatter(int lanes, const void* in, void* out, const Bucke
               (uint64_t)fan->W * fan->per < ((uint64_t)nseg << digit_bits)))
     return hipErrorInvalidValue;
   const uint64_t total = (uint64_t)nseg * chunks;
-  dim3 grid(total < 65536 ? (uint32_t)total : 65536u);
+  // a whole number of workgroups per XCD (the kernel's XCD-aware order)
+  uint32_t g = total < 65536 ? (uint32_t)total : 65536u;
+  if (nseg >= (uint32_t)kXcds && g >= (uint32_t)kXcds) g -= g % kXcds;
+  dim3 grid(g);
   const bool narrow = digit_bits <= (uint32_t)kNarrowDigitBits;
   const bool mid = digit_bits <= (uint32_t)kMidDigitBits;
   const Runs rn = runs ? *runs : Runs{nullptr, nullptr, 0};
