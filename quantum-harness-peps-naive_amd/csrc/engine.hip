@@ -793,7 +793,7 @@ int Engine::pipeline(const RunSpec& spec, Plan* plan, bool eager, RunOut* out, s
   const uint32_t rbits = std::max<uint32_t>(1, std::min<uint32_t>(spec.radix_bits, kMaxDigitBits));
   // LDS merge table: 2048 slots (more workgroups per CU) for small tiles,
   // 4096 slots otherwise; the bucket target stays within one merge pass.
-  const uint32_t table_slots = spec.tile_states <= 1024 ? 2048u : 4096u;
+  const uint32_t table_slots = spec.tile_states <= merge_cap(2048) ? 2048u : 4096u;
   // Buckets may span several LDS passes: the merge re-reads the bucket per
   // pass (from L2/MALL), which can be cheaper than one more partition level.
   const uint64_t target = (uint64_t)std::max<uint32_t>(256, std::min<uint32_t>(spec.tile_states, merge_cap(table_slots))) *
@@ -1475,7 +1475,7 @@ int Engine::materialized(const RunSpec& spec, RunOut* out, std::string* err) {
   const uint64_t final_mask = g0.board_mask;
 
   const uint32_t rbits = std::max<uint32_t>(1, std::min<uint32_t>(spec.radix_bits, kMaxDigitBits));
-  const uint32_t table_slots = spec.tile_states <= 1024 ? 2048u : 4096u;
+  const uint32_t table_slots = spec.tile_states <= merge_cap(2048) ? 2048u : 4096u;
   const uint64_t target = (uint64_t)std::max<uint32_t>(256, std::min<uint32_t>(spec.tile_states, merge_cap(table_slots)));
   const uint64_t merge_grid = (uint64_t)cu_count_ * merge_resident_blocks(L, table_slots, kFineBits) *
                               std::max<uint32_t>(1, std::min<uint32_t>(spec.merge_waves, 8));

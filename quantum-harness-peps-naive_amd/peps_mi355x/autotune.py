@@ -1,7 +1,7 @@
 """Autotuner for the contraction's launch shape.
 
 Sweeps the engine's tunables -- hash-radix digit width per partition level
-(``radix_bits``), final bucket size (``tile_states``; up to 1024 records use a
+(``radix_bits``), final bucket size (``tile_states``; up to 1280 records use a
 2048-slot LDS table, more a 4096-slot table), the digit width the merge-expand
 scatter writes (``expand_bits``), the merge grid (``merge_waves``) and, for
 the batched form, the slice grouping (``slice_group``) -- times each shape on
