@@ -219,7 +219,7 @@ def serve(args, pm):
     info = pm.device_info(0)
     log(f"device: {info['name']} CUs={info['cu_count']}")
     sweep = []
-    with pm.Server() as server:
+    with pm.Server(policy=args.policy) as server:
         # warm the engine (arena, code objects) on a short stream
         warm = sv.make_requests(args.seed + 1, 50, 1e6, args.serve_n_min, args.serve_n_max)
         sv.run_open_loop(server, warm)
@@ -264,7 +264,7 @@ def serve(args, pm):
                  f"Poisson arrivals, N uniform in [{args.serve_n_min}, {args.serve_n_max}], "
                  f"random non-attacking prefix of 0..2 rows"),
         "config": {"workload": "exact prefix-completion counts through the request server",
-                   "requests": args.requests, "rates": args.rates,
+                   "requests": args.requests, "rates": args.rates, "policy": args.policy,
                    "n_range": [args.serve_n_min, args.serve_n_max]},
         "p50_s": best["p50_s"],
         "p99_s": best["p99_s"],
@@ -286,6 +286,8 @@ def main():
     ap.add_argument("--rates", default="100,400,1600,100000")
     ap.add_argument("--serve-n-min", type=int, default=10)
     ap.add_argument("--serve-n-max", type=int, default=16)
+    ap.add_argument("--policy", default="response_ratio", choices=("shortest", "fifo", "response_ratio"),
+                    help="request server scheduling policy")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--lanes", type=int, default=0, help="CRT lanes (0: the plan for N)")
     ap.add_argument("--graph", action="store_true",

@@ -339,7 +339,10 @@ int peps_sha256(const void* data, size_t len, uint8_t out[32]);
  * the reference computes one count per process (src/main.rs:25-70). */
 typedef struct peps_server peps_server;
 
-enum { PEPS_POLICY_SHORTEST_FIRST = 0, PEPS_POLICY_FIFO = 1 };
+/* Which request's class the worker serves next: least projected remaining
+ * work, oldest, or highest response ratio (wait + projected service time) /
+ * projected service time -- short requests first, long ones aged in. */
+enum { PEPS_POLICY_SHORTEST_FIRST = 0, PEPS_POLICY_FIFO = 1, PEPS_POLICY_RESPONSE_RATIO = 2 };
 
 typedef struct {
   uint64_t ticket;

@@ -95,6 +95,7 @@ struct peps_server {
   uint64_t next_id = 1;
   uint64_t pending = 0, submitted = 0, completed = 0, failed = 0, launches = 0, tasks_run = 0, retries = 0;
   double busy_s = 0;
+  double sec_per_work = 0;  // learned launch seconds per unit of projected work
   bool stop = false;
   std::thread worker;
 
@@ -115,18 +116,29 @@ struct peps_server {
     done_cv.notify_all();
   }
 
-  // The request whose class is served next (mu held, runnable non-empty).
+  // The request whose class is served next (mu held, runnable non-empty):
+  // the oldest (FIFO), the one with the least projected remaining work
+  // (shortest first), or the one with the highest response ratio
+  // (wait + service) / service, service = projected work at the learned
+  // seconds per unit of work -- short requests still go first, and a long
+  // one's priority grows with its wait, so it cannot starve.
   Request* leader() {
     Request* best = nullptr;
-    double best_work = 0;
+    double best_key = 0;
+    const double t = now();
     for (uint64_t id : runnable) {
       Request& r = requests.at(id);
       if (policy == PEPS_POLICY_FIFO) return &r;
       const double ratio = projection.ratio(r.n, r.depth);
       const double work = r.remaining * (ratio > 0 ? ratio : 1.0);
-      if (!best || work < best_work) {
+      double key = work;
+      if (policy == PEPS_POLICY_RESPONSE_RATIO && sec_per_work > 0) {
+        const double svc = std::max(work * sec_per_work, 1e-4);
+        key = -(t - r.t_submit + svc) / svc;
+      }
+      if (!best || key < best_key) {
         best = &r;
-        best_work = work;
+        best_key = key;
       }
     }
     return best;
@@ -225,6 +237,14 @@ struct peps_server {
 
       lk.lock();
       busy_s += t_end - t_launch;
+      if (rc == PEPS_OK && ratio > 0) {
+        double work = 0;
+        for (double x : bproxy) work += x * ratio;
+        if (work > 0) {
+          const double spw = (t_end - t_launch) / work;
+          sec_per_work = sec_per_work > 0 ? 0.8 * sec_per_work + 0.2 * spw : spw;
+        }
+      }
       launches += ranges.size();
       tasks_run += batch.size();
       for (size_t j = 0; j < ranges.size(); ++j) {
@@ -290,7 +310,7 @@ int peps_server_create(const peps_options* opts, uint32_t policy, uint32_t max_p
                        peps_server** out, char* err, size_t err_cap) {
   if (!out) return fail(PEPS_E_INVALID, "null server pointer", err, err_cap);
   *out = nullptr;
-  if (policy > PEPS_POLICY_FIFO) return fail(PEPS_E_INVALID, "unknown scheduling policy", err, err_cap);
+  if (policy > PEPS_POLICY_RESPONSE_RATIO) return fail(PEPS_E_INVALID, "unknown scheduling policy", err, err_cap);
   std::string e;
   peps::Resolved probe;
   peps_options o{};

@@ -23,6 +23,7 @@ PEPS_E_INTERNAL = 8
 
 PEPS_POLICY_SHORTEST_FIRST = 0
 PEPS_POLICY_FIFO = 1
+PEPS_POLICY_RESPONSE_RATIO = 2
 
 PEPS_ROW_OPERATOR_COMPILED = 0
 PEPS_ROW_OPERATOR_SITEWISE = 1

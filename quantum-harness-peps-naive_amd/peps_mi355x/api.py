@@ -20,6 +20,7 @@ from .abi import (
     PEPS_E_INVALID,
     PEPS_E_PENDING,
     PEPS_POLICY_FIFO,
+    PEPS_POLICY_RESPONSE_RATIO,
     PEPS_POLICY_SHORTEST_FIRST,
     PEPS_SYMMETRY_NONE,
     PEPS_SYMMETRY_TOP_ROW_ORBITS,
@@ -323,8 +324,9 @@ class Server:
     batching); a result does not depend on what else was in flight.
     """
 
-    def __init__(self, options: PepsOptions = None, policy: str = "shortest", max_pending: int = 0):
-        pol = {"shortest": PEPS_POLICY_SHORTEST_FIRST, "fifo": PEPS_POLICY_FIFO}[policy]
+    def __init__(self, options: PepsOptions = None, policy: str = "response_ratio", max_pending: int = 0):
+        pol = {"shortest": PEPS_POLICY_SHORTEST_FIRST, "fifo": PEPS_POLICY_FIFO,
+               "response_ratio": PEPS_POLICY_RESPONSE_RATIO}[policy]
         self._h = ctypes.c_void_p()
         e = _err()
         ref, _keep = _opts(options)

@@ -83,7 +83,7 @@ def test_server_rejects_bad_policy():
 # ---------------------------------------------------------------- GPU side --
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("policy", ["shortest", "fifo"])
+@pytest.mark.parametrize("policy", ["shortest", "fifo", "response_ratio"])
 def test_mix_matches_oracle_and_alone(policy):
     reqs = sv.make_requests(seed=0, count=80, rate=1e6, n_min=6, n_max=13)
     with pm.Server(policy=policy) as s:
