@@ -11,7 +11,8 @@ time is the wall time per count with the device warmed.
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 18]
                        [--form plain|batched] [--no-cpu-baseline] [--no-serving]
        python bench.py --form serving [--requests 2000] [--rates 50,200,800]
-                       [--serve-n-min 10] [--serve-n-max 16]
+                       [--serve-n-min 10] [--serve-n-max 16] [--policy response_ratio]
+       python bench.py --form cpu [--n 18] [--task-range a:b]   (CPU restatement only)
 
 The plain form (the default) also streams a bounded mixed-N request set
 (300 requests, N uniform in [10, 18], 0..2 placed rows, saturating arrivals)
@@ -166,6 +167,37 @@ def cpu_baseline(n, budget_s):
     }
 
 
+def cpu_full(args):
+    """The oracle's restatement of the reference's rayon path over every
+    2-row prefix task of N (or the range --task-range a:b of them), in task
+    order, timed per task: the full CPU count that the sampled cpu_baseline
+    extrapolates.  Prints one JSON line."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import _oracle
+
+    n = args.n
+    threads = min(host_threads(), 64)
+    tasks = [(c0, c1) for c0 in range((n + 1) // 2) for c1 in range(n)
+             if c1 != c0 and abs(c1 - c0) != 1]
+    a, b = 0, len(tasks)
+    if args.task_range:
+        a, b = (int(x) for x in args.task_range.split(":"))
+    count, matched, secs, per = 0, 0, 0.0, []
+    for i in range(a, b):
+        t0 = time.perf_counter()
+        r = _oracle.contract_prefix(n, list(tasks[i]), threads=threads)
+        dt = time.perf_counter() - t0
+        w = 1 if tasks[i][0] == n - 1 - tasks[i][0] else 2
+        count += w * r.count.value
+        matched += r.row_operator_matched.value
+        secs += dt
+        per.append(round(dt, 3))
+        log(f"cpu task {i} {tasks[i]}: {dt:.2f}s (total {secs:.1f}s)")
+    print(json.dumps({"form": "cpu", "n": n, "tasks": [a, b], "of_tasks": len(tasks), "seconds": secs,
+                      "weighted_count": count, "transitions": matched, "threads": threads,
+                      "cpu": host_cpu_model(), "per_task_s": per}), flush=True)
+
+
 def serving_leg(pm, requests, n_min, n_max, seed, rate):
     """A bounded mixed-N request stream through the request server after the
     timed region: open-loop arrivals at `rate`, then the same requests as
@@ -281,7 +313,8 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=18)
-    ap.add_argument("--form", choices=("plain", "batched", "serving"), default="plain")
+    ap.add_argument("--form", choices=("plain", "batched", "serving", "cpu"), default="plain")
+    ap.add_argument("--task-range", default="", help="--form cpu: tasks a:b of the 2-row prefix tasks")
     ap.add_argument("--requests", type=int, default=2000)
     ap.add_argument("--rates", default="100,400,1600,100000")
     ap.add_argument("--serve-n-min", type=int, default=10)
@@ -315,6 +348,9 @@ def main():
               f"and only the one-GPU throughput is measured",
               file=sys.stderr)
         sys.exit(2)
+
+    if args.form == "cpu":
+        return cpu_full(args)
 
     import peps_mi355x as pm
 
