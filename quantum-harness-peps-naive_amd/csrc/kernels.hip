@@ -1200,7 +1200,10 @@ __global__ __launch_bounds__(BT, (BT == kScatterBlock && L <= 2 && T == 2048 && 
   const uint32_t W = gridDim.x;
   const int lane = threadIdx.x & 63;
 
-  for (uint32_t i = threadIdx.x; i < T; i += BT) tkey[i] = kEmptyKey;
+  for (uint32_t i = threadIdx.x; i < T; i += BT) {
+    tkey[i] = kEmptyKey;
+    if constexpr (P && MODE != kMergeCount) tw[i] = 0;
+  }
   if (kExpand) {
     for (uint32_t d = threadIdx.x; d < D; d += BT) {
       lcount[d] = 0;
@@ -1308,39 +1311,38 @@ __global__ __launch_bounds__(BT, (BT == kScatterBlock && L <= 2 && T == 2048 && 
             ulist[base_idx + r] = (uint16_t)sl[k];
           }
         }
-        if (MODE != kMergeCount) {
+        if constexpr (P && MODE != kMergeCount) {
+          // ---- 2+3. packed: every record adds its exact weight into its
+          // slot's sum (zero while the slot is free), owner or duplicate
+          // alike, so no barrier has to order the owners' stores first (a
+          // bucket's sum of weights below 2^52 stays below 2^64).
+#pragma unroll
+          for (int k = 0; k < KB; ++k)
+            if (((own | dup) >> k) & 1u)
+              atomicAdd((unsigned long long*)&tw[sl[k]], (unsigned long long)packed_weight(buf[k], g.key_bits));
+        } else if constexpr (!P && MODE != kMergeCount) {
           lds_barrier();
-          // ---- 2. owners store their residues (packed: exact weights)
+          // ---- 2. owners store their residues
 #pragma unroll
           for (int k = 0; k < KB; ++k)
             if ((own >> k) & 1u) {
-              if constexpr (P) {
-                tw[sl[k]] = packed_weight(buf[k], g.key_bits);
-              } else {
 #pragma unroll
-                for (int l = 0; l < L; ++l) tres[l < LL ? l : 0][sl[k]] = buf[k].r[l];
-              }
+              for (int l = 0; l < NL; ++l) tres[l < LL ? l : 0][sl[k]] = buf[k].r[l];
             }
           lds_barrier();
-          // ---- 3. duplicates add modulo each prime (packed: plain 64-bit
-          // adds; a bucket's sum of weights below 2^52 stays below 2^64)
+          // ---- 3. duplicates add modulo each prime
           if (dup) {
 #pragma unroll
             for (int k = 0; k < KB; ++k)
               if ((dup >> k) & 1u) {
-                if constexpr (P) {
-                  atomicAdd((unsigned long long*)&tw[sl[k]],
-                            (unsigned long long)packed_weight(buf[k], g.key_bits));
-                } else {
 #pragma unroll
-                  for (int l = 0; l < L; ++l) {
-                    uint32_t* a = &tres[l < LL ? l : 0][sl[k]];
-                    uint32_t old = *a, assumed;
-                    do {
-                      assumed = old;
-                      old = atomicCAS(a, assumed, addmod(assumed, buf[k].r[l], pr.p[l]));
-                    } while (old != assumed);
-                  }
+                for (int l = 0; l < NL; ++l) {
+                  uint32_t* a = &tres[l < LL ? l : 0][sl[k]];
+                  uint32_t old = *a, assumed;
+                  do {
+                    assumed = old;
+                    old = atomicCAS(a, assumed, addmod(assumed, buf[k].r[l], pr.p[l]));
+                  } while (old != assumed);
                 }
               }
           }
@@ -1361,6 +1363,7 @@ __global__ __launch_bounds__(BT, (BT == kScatterBlock && L <= 2 && T == 2048 && 
         if (MODE != kMergeCount) {
           if constexpr (P) {
             wv = tw[slot];
+            tw[slot] = 0;  // the slot is free again
             if (wv >> g.weight_bits) overflow |= 4u;
             wsum[0] += wv;
             wsum[1] += wsum[0] < wv;
