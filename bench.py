@@ -43,11 +43,11 @@ ROCPROF_PATTERNS = {
     "merge_scatter": r"k_merge_expand<{L}, 1, \d+, \d+, \d+>",
     "merge_last": r"k_merge_expand<{L}, 2, \d+, \d+, \d+>",
     "count_ub": r"k_count_ub<{L}>",
-    "part_hist": r"k_part_hist<{L}, \d+, false, false>",
-    "part_hist_runs": r"k_part_hist<{L}, \d+, true, false>",
-    "part_hist_ub": r"k_part_hist<{L}, \d+, false, true>",
-    "part_scatter": r"k_part_scatter<{L}, \d+, false>",
-    "part_scatter_runs": r"k_part_scatter<{L}, \d+, true>",
+    "caps": r"k_caps_(scan|expand)",
+    "part_hist": r"k_part_hist<{L}, \d+, false>",
+    "part_hist_runs": r"k_part_hist<{L}, \d+, true>",
+    "part_scatter": r"k_part_scatter<{L}, \d+, false, (true|false), (true|false)>",
+    "part_scatter_runs": r"k_part_scatter<{L}, \d+, true, (true|false), (true|false)>",
 }
 
 
@@ -258,7 +258,7 @@ def serve(args, pm):
         for rate in [float(x) for x in args.rates.split(",")]:
             reqs = sv.make_requests(args.seed, args.requests, rate, args.serve_n_min, args.serve_n_max)
             st0 = server.stats()
-            outs = sv.run_open_loop(server, reqs, timeout_s=1800,
+            outs = sv.run_open_loop(server, reqs, timeout_s=1800, every=args.progress_every,
                                     progress=lambda stage, d, t: log(f"rate {rate:g}: {stage} {d}/{t}"))
             st1 = server.stats()
             bad = [(n, c, o.count) for (_, n, c), o in zip(reqs, outs)
@@ -274,9 +274,12 @@ def serve(args, pm):
         reqs = sv.make_requests(args.seed, args.requests, 1.0, args.serve_n_min, args.serve_n_max)
         t0 = time.perf_counter()
         for i, (_, n, cols) in enumerate(reqs):
-            pm.count_prefix(n, cols)
-            if (i + 1) % 500 == 0:
-                log(f"back-to-back: {i + 1}/{len(reqs)}")
+            if not cols and n >= 19:
+                pm.count_batched(n, 2)  # the empty board's plain frontier does not fit one GPU
+            else:
+                pm.count_prefix(n, cols)
+            if (i + 1) % args.progress_every == 0 or n >= 19:
+                log(f"back-to-back: {i + 1}/{len(reqs)} (N={n})")
         b2b_s = time.perf_counter() - t0
     best = max(sweep, key=lambda x: x["requests_per_s"])
     b2b_rate = args.requests / b2b_s
@@ -319,6 +322,8 @@ def main():
     ap.add_argument("--rates", default="100,400,1600,100000")
     ap.add_argument("--serve-n-min", type=int, default=10)
     ap.add_argument("--serve-n-max", type=int, default=16)
+    ap.add_argument("--progress-every", type=int, default=250,
+                    help="serving: report every this many submitted / finished requests")
     ap.add_argument("--policy", default="response_ratio", choices=("shortest", "fifo", "response_ratio"),
                     help="request server scheduling policy")
     ap.add_argument("--seed", type=int, default=0)

@@ -73,10 +73,12 @@ def percentile(sorted_values, num: int, den: int):
     return sorted_values[(len(sorted_values) - 1) * num // den]
 
 
-def run_open_loop(server, requests, timeout_s: float = 600.0, progress=None):
+def run_open_loop(server, requests, timeout_s: float = 600.0, progress=None, every: int = 250):
     """Submit each request at its arrival time (relative to now); return the outcomes in order.
 
-    ``progress(stage, done, total)`` is called every 250 submissions and results."""
+    ``progress(stage, done, total)`` is called every ``every`` submissions,
+    and whenever requests finish or the server completes more engine
+    launches (at most twice a second), so a long stream keeps reporting."""
     tickets = [None] * len(requests)
     t0 = time.perf_counter()
     for i, (arrival, n, cols) in enumerate(requests):
@@ -84,14 +86,29 @@ def run_open_loop(server, requests, timeout_s: float = 600.0, progress=None):
         if delay > 0:
             time.sleep(delay)
         tickets[i] = server.submit(n, cols)
-        if progress and (i + 1) % 250 == 0:
+        if progress and (i + 1) % every == 0:
             progress("submitted", i + 1, len(requests))
     deadline = time.perf_counter() + timeout_s
-    out = []
-    for i, t in enumerate(tickets):
-        out.append(server.wait(t, max(0.0, deadline - time.perf_counter())))
-        if progress and (i + 1) % 250 == 0:
-            progress("finished", i + 1, len(requests))
+    out = [None] * len(tickets)
+    left = set(range(len(tickets)))
+    reported, launches = 0, None
+    while left:
+        for i in sorted(left):
+            r = server.poll(tickets[i])
+            if r is not None:
+                out[i] = r
+                left.discard(i)
+        done = len(tickets) - len(left)
+        if progress:
+            st = server.stats()["launches"]
+            if done // every > reported // every or (st != launches and done < len(tickets)):
+                progress(f"finished (engine launches {st})", done, len(tickets))
+                reported, launches = done, st
+        if not left:
+            break
+        if time.perf_counter() > deadline:
+            raise TimeoutError(f"{len(left)} requests not finished after {timeout_s} s")
+        time.sleep(0.5)
     return out
 
 
