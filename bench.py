@@ -247,6 +247,7 @@ def serving_leg(pm, requests, n_min, n_max, seed, rate):
 def serve(args, pm):
     """Open-loop request streams through the request server, one per rate."""
     from peps_mi355x import serving as sv
+    from peps_mi355x.abi import PEPS_E_MEMORY
 
     info = pm.device_info(0)
     log(f"device: {info['name']} CUs={info['cu_count']}")
@@ -272,12 +273,26 @@ def serve(args, pm):
             sweep.append(summ)
         # The same requests back to back, one plain GPU count each (no batching).
         reqs = sv.make_requests(args.seed, args.requests, 1.0, args.serve_n_min, args.serve_n_max)
+
+        def plain(n, cols):
+            # A prefix whose frontier outgrows the arena is counted as the sum
+            # of its one-row extensions, each a plain count of its own.
+            try:
+                return pm.count_prefix(n, cols)
+            except pm.PepsError as e:
+                if e.code != PEPS_E_MEMORY:
+                    raise
+            taken = set(cols)
+            r = len(cols)
+            return sum(plain(n, cols + [c]) for c in range(n)
+                       if c not in taken and all(abs(c - p) != r - i for i, p in enumerate(cols)))
+
         t0 = time.perf_counter()
         for i, (_, n, cols) in enumerate(reqs):
             if not cols and n >= 19:
                 pm.count_batched(n, 2)  # the empty board's plain frontier does not fit one GPU
             else:
-                pm.count_prefix(n, cols)
+                plain(n, list(cols))
             if (i + 1) % args.progress_every == 0 or n >= 19:
                 log(f"back-to-back: {i + 1}/{len(reqs)} (N={n})")
         b2b_s = time.perf_counter() - t0

@@ -182,6 +182,24 @@ __device__ __forceinline__ uint64_t successor(uint64_t key, const Board& s, uint
   return (key & ~g.state_mask) | (uint64_t)c | ((uint64_t)dr << g.n) | ((uint64_t)dl << (2 * g.n));
 }
 
+// The same successor split into a per-state part and a per-column part:
+// at a legal column b every incoming leg equals the occupied entry's input
+// leg, so writing the output legs there flips exactly the bits of
+// b & (in ^ out) in each signal, and the shifts distribute over the flips.
+// successor(key, s, b, g) == successor_base(key, s, g) ^ successor_delta(b, g)
+// for every b in legal_positions(s, g).
+__device__ __forceinline__ uint64_t successor_base(uint64_t key, const Board& s, const RowGeom& g) {
+  return (key & ~g.state_mask) | (uint64_t)s.cols | ((uint64_t)((s.dr << 1) & g.board_mask) << g.n) |
+         ((uint64_t)(s.dl >> 1) << (2 * g.n));
+}
+
+__device__ __forceinline__ uint64_t successor_delta(uint32_t b, const RowGeom& g) {
+  const uint32_t c = b & (g.col_in ^ g.col_out);
+  const uint32_t dr = ((b & (g.dr_in ^ g.dr_out)) << 1) & g.board_mask;
+  const uint32_t dl = (b & (g.dl_in ^ g.dl_out)) >> 1;
+  return (uint64_t)c | ((uint64_t)dr << g.n) | ((uint64_t)dl << (2 * g.n));
+}
+
 // The top `bits` bits of mix64(key), for bits <= 29 (callers take at most
 // kFineBits).  The final xorshift by 29 leaves the product's top 29 bits as
 // they are, so only the product's high word is formed: with k = hi:lo and
@@ -1187,7 +1205,10 @@ __global__ __launch_bounds__(BT, (BT == kScatterBlock && L <= 2 && T == 2048 && 
   __shared__ uint64_t tw[(P && MODE != kMergeCount) ? T : 1];  // packed: exact weights
   __shared__ uint16_t ulist[T];
   __shared__ uint32_t lcount[DM];
+  // SCATTER: byte offset in `out` of this workgroup's run of each digit.
   __shared__ uint64_t gpos[MODE == kMergeScatter ? DM : 1];
+  // successor_delta of every column (index: the column's bit position).
+  __shared__ uint64_t dtab[kExpand ? 32 : 1];
   __shared__ uint64_t scratch[BT / 64];
   __shared__ uint32_t nuniq[2];
   // Last row: per-slice sums gathered in LDS, flushed with one global atomic
@@ -1207,8 +1228,10 @@ __global__ __launch_bounds__(BT, (BT == kScatterBlock && L <= 2 && T == 2048 && 
   if (kExpand) {
     for (uint32_t d = threadIdx.x; d < D; d += BT) {
       lcount[d] = 0;
-      if (MODE == kMergeScatter) gpos[d] = offsets[(uint64_t)d * W + blockIdx.x];
+      if (MODE == kMergeScatter)
+        gpos[d] = offsets[(uint64_t)d * W + blockIdx.x] * sizeof(Rec<L>);
     }
+    if (threadIdx.x < 32) dtab[threadIdx.x] = threadIdx.x < g.n ? successor_delta(1u << threadIdx.x, g) : 0;
   }
   if (threadIdx.x < 2) nuniq[threadIdx.x] = 0;
   if (lds_slices)
@@ -1252,7 +1275,9 @@ __global__ __launch_bounds__(BT, (BT == kScatterBlock && L <= 2 && T == 2048 && 
   // Capped SCATTER runs: at most wcap successors per digit (more sets bit 4
   // of counters[kRetry] and writes nothing past the run; the engine then
   // repeats the SCATTER with upper-bound offsets).
-  const uint64_t wcap = (MODE == kMergeScatter && capw) ? capw[blockIdx.x] : ~0ull;
+  const uint64_t wcap64 = (MODE == kMergeScatter && capw) ? capw[blockIdx.x] : ~0ull;
+  const uint32_t wcap = wcap64 < 0xffffffffull ? (uint32_t)wcap64 : 0xffffffffu;
+  const uint64_t out_end = cap * sizeof(Rec<L>);  // bytes of the output region
 
   for (uint32_t b = b0; b < b1; ++b) {
     const uint64_t lo = plo, hi = phi;
@@ -1426,15 +1451,18 @@ __global__ __launch_bounds__(BT, (BT == kScatterBlock && L <= 2 && T == 2048 && 
         if (kExpand) {
           const Board bs = unpack(key, g);
           uint32_t free = legal_positions(bs, g);
+          const uint64_t sbase = successor_base(key, bs, g);
+          // The next column's delta is read one successor ahead.
+          uint64_t delta = free ? dtab[__builtin_ctz(free)] : 0;
           while (free) {
-            const uint32_t bit = free & (~free + 1);
-            free ^= bit;
-            const uint64_t sk = successor(key, bs, bit, g);
+            const uint64_t sk = sbase ^ delta;
+            free &= free - 1;
+            if (free) delta = dtab[__builtin_ctz(free)];
             const uint32_t d = top_digit(sk, digit_bits);
             if (MODE == kMergeCount) {
               atomicAdd(&lcount[d], 1u);
             } else {
-              // The run's start is read before the rank is taken, so both
+              // The run's offset is read before the rank is taken, so both
               // LDS accesses are in flight together.
               const uint64_t run0 = gpos[d];
               const uint32_t rank = atomicAdd(&lcount[d], 1u);
@@ -1446,10 +1474,10 @@ __global__ __launch_bounds__(BT, (BT == kScatterBlock && L <= 2 && T == 2048 && 
 #pragma unroll
                 for (int l = 0; l < L; ++l) rec.r[l] = v[l];
               }
-              const uint64_t pos = run0 + rank;
-              const bool in_run = rank < wcap, in_region = pos < cap;
+              const uint64_t at = run0 + (uint64_t)rank * sizeof(Rec<L>);
+              const bool in_run = rank < wcap, in_region = at < out_end;
               if (in_run && in_region)
-                out[pos] = rec;
+                *(Rec<L>*)((char*)out + at) = rec;
               else
                 overflow |= in_run ? 2u : 16u;
             }
