@@ -111,8 +111,8 @@ def cpu_baseline(n, budget_s):
     a bounded sample: whole 2-row prefix tasks of the same board, stratified
     over the first-row column (centre and edges first, then the columns
     between), timed until the budget is spent.  Rate = transitions
-    (row_operator_matched) / s over the sampled tasks; the full count's time
-    is estimated as the plain count's transitions at that rate."""
+    (row_operator_matched) / s over the sampled tasks; the whole CPU count's
+    time is estimated as the sum of every task's transitions at that rate."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import _oracle
 
@@ -200,9 +200,11 @@ def cpu_full(args):
 
 def serving_leg(pm, requests, n_min, n_max, seed, rate):
     """A bounded mixed-N request stream through the request server after the
-    timed region: open-loop arrivals at `rate`, then the same requests as
-    back-to-back plain GPU counts.  Every served count must equal its
-    back-to-back count (and known_count for an empty prefix)."""
+    timed region: open-loop arrivals at `rate` (saturating by default: the
+    throughput), the same requests again paced at half the throughput that
+    stream reached (the latency a caller sees below capacity), then the same
+    requests as back-to-back plain GPU counts.  Every served count must equal
+    its back-to-back count (and known_count for an empty prefix)."""
     from peps_mi355x import serving as sv
 
     reqs = sv.make_requests(seed, requests, rate, n_min, n_max)
@@ -212,6 +214,12 @@ def serving_leg(pm, requests, n_min, n_max, seed, rate):
         outs = sv.run_open_loop(server, reqs, timeout_s=600,
                                 progress=lambda stage, d, t: log(f"serving: {stage} {d}/{t}"))
         st1 = server.stats()
+        summ = sv.summarize(outs)
+        paced_rate = 0.5 * summ["requests_per_s"]
+        paced_reqs = sv.make_requests(seed, requests, paced_rate, n_min, n_max)
+        paced_outs = sv.run_open_loop(server, paced_reqs, timeout_s=600,
+                                      progress=lambda stage, d, t: log(f"serving (paced): {stage} {d}/{t}"))
+        paced = sv.summarize(paced_outs)
     t0 = time.perf_counter()
     alone = []
     for i, (_, n, cols) in enumerate(reqs):
@@ -219,11 +227,11 @@ def serving_leg(pm, requests, n_min, n_max, seed, rate):
         if (i + 1) % 100 == 0:
             log(f"back-to-back: {i + 1}/{len(reqs)}")
     b2b_s = time.perf_counter() - t0
-    bad = [(n, c) for (_, n, c), o, a in zip(reqs, outs, alone)
-           if o.status != 0 or o.count != a or (not c and a != pm.known_count(n))]
+    bad = [(n, c) for (_, n, c), o, p, a in zip(reqs, outs, paced_outs, alone)
+           if o.status != 0 or p.status != 0 or o.count != a or p.count != a
+           or (not c and a != pm.known_count(n))]
     if bad:
         raise SystemExit(f"serving: wrong or failed requests: {bad[:5]}")
-    summ = sv.summarize(outs)
     b2b_rate = len(reqs) / b2b_s
     return {
         "requests": len(reqs),
@@ -238,6 +246,9 @@ def serving_leg(pm, requests, n_min, n_max, seed, rate):
         "max_s": summ["max_s"],
         "percentile_rule": "sorted[(len - 1) * q] (src/bin/dfs_bitmask.rs:70-73)",
         "launches": st1["launches"] - st0["launches"],
+        "paced": {"offered_rate": paced_rate, "requests_per_s": paced["requests_per_s"],
+                  "p50_s": paced["p50_s"], "p90_s": paced["p90_s"], "p99_s": paced["p99_s"],
+                  "max_s": paced["max_s"]},
         "back_to_back_requests_per_s": b2b_rate,
         "vs_back_to_back": summ["requests_per_s"] / b2b_rate,
         "checked": "every count equals the same request as a plain GPU count; empty prefixes equal known_count",
@@ -520,10 +531,22 @@ def main():
     if serving:
         # mixed-N serving after the timed region (extra keys; `value` is the count rate above)
         out.update({"p50_s": serving["p50_s"], "p99_s": serving["p99_s"],
+                    "p50_s_paced": serving["paced"]["p50_s"], "p99_s_paced": serving["paced"]["p99_s"],
                     "requests_per_s": serving["requests_per_s"],
                     "vs_back_to_back": serving["vs_back_to_back"], "serving": serving})
     if cpu and cpu.get("value"):
-        out["cpu_time_to_count_est_s"] = plain_matched / cpu["value"]
+        # The CPU path counts the board as its 2-row prefix tasks, whose
+        # frontiers do not merge with each other: the whole job is the sum of
+        # the tasks' transitions (the batched GPU form's counter), at the
+        # sampled rate.
+        try:
+            rb, ntasks = pm.count_batched(n, 2)
+            job = rb.row_operator_matched.value
+            cpu["job_transitions"] = job
+            cpu["job_tasks"] = ntasks
+            out["cpu_time_to_count_est_s"] = job / cpu["value"]
+        except pm.PepsError as e:
+            log(f"task transitions unavailable ({e})")
     print(json.dumps(out), flush=True)
 
 
