@@ -1238,7 +1238,7 @@ __global__ __launch_bounds__(BT, (BT == kScatterBlock && L <= 2 && T == 2048 && 
     for (uint32_t i = threadIdx.x; i < nslices * NL; i += BT) ssl[i] = 0;
   lds_barrier();
 
-  uint64_t uniq = 0;
+  uint32_t uniq = 0;  // merged states of this thread
   // Lane records: per-thread residue sums.  Packed: exact 128-bit sums
   // (index 0 low, 1 high) reduced to residues at the end.
   uint64_t wsum[P ? 2 : L], fsum[P ? 2 : L];
@@ -1400,8 +1400,10 @@ __global__ __launch_bounds__(BT, (BT == kScatterBlock && L <= 2 && T == 2048 && 
             }
           }
           // Complete boundaries (every column set) end here: the final v1/v2
-          // contraction sums them, per slice when slices are tracked.
-          if (final_mask && (key & final_mask) == final_mask) {
+          // contraction sums them, per slice when slices are tracked.  Only
+          // the last row's merge (LAST, or EMIT on the materialised path)
+          // can hold them: every row places one column.
+          if (MODE != kMergeScatter && final_mask && (key & final_mask) == final_mask) {
             const uint64_t sl = (key & ~g.state_mask) >> (3 * g.n);
             if constexpr (P) {
               fsum[0] += wv;
@@ -1452,34 +1454,46 @@ __global__ __launch_bounds__(BT, (BT == kScatterBlock && L <= 2 && T == 2048 && 
           const Board bs = unpack(key, g);
           uint32_t free = legal_positions(bs, g);
           const uint64_t sbase = successor_base(key, bs, g);
-          // The next column's delta is read one successor ahead.
-          uint64_t delta = free ? dtab[__builtin_ctz(free)] : 0;
-          while (free) {
-            const uint64_t sk = sbase ^ delta;
-            free &= free - 1;
-            if (free) delta = dtab[__builtin_ctz(free)];
-            const uint32_t d = top_digit(sk, digit_bits);
-            if (MODE == kMergeCount) {
-              atomicAdd(&lcount[d], 1u);
+          // Two successors per step: their table reads, run offsets and rank
+          // atomics are in flight together.
+          auto emit = [&](uint64_t sk, uint64_t run0, uint32_t rank) {
+            Rec<L> rec;
+            if constexpr (P) {
+              rec = make_packed(sk, wv, g.key_bits);
             } else {
-              // The run's offset is read before the rank is taken, so both
-              // LDS accesses are in flight together.
-              const uint64_t run0 = gpos[d];
-              const uint32_t rank = atomicAdd(&lcount[d], 1u);
-              Rec<L> rec;
-              if constexpr (P) {
-                rec = make_packed(sk, wv, g.key_bits);
-              } else {
-                rec.key = sk;
+              rec.key = sk;
 #pragma unroll
-                for (int l = 0; l < L; ++l) rec.r[l] = v[l];
-              }
-              const uint64_t at = run0 + (uint64_t)rank * sizeof(Rec<L>);
-              const bool in_run = rank < wcap, in_region = at < out_end;
-              if (in_run && in_region)
-                *(Rec<L>*)((char*)out + at) = rec;
-              else
-                overflow |= in_run ? 2u : 16u;
+              for (int l = 0; l < L; ++l) rec.r[l] = v[l];
+            }
+            const uint64_t at = run0 + (uint64_t)rank * sizeof(Rec<L>);
+            const bool in_run = rank < wcap, in_region = at < out_end;
+            if (in_run && in_region)
+              *(Rec<L>*)((char*)out + at) = rec;
+            else
+              overflow |= in_run ? 2u : 16u;
+          };
+          while (free) {
+            const uint32_t c0 = (uint32_t)__builtin_ctz(free);
+            free &= free - 1;
+            const bool two = free != 0;
+            const uint32_t c1 = two ? (uint32_t)__builtin_ctz(free) : c0;
+            free &= free - 1;  // no-op when free is already 0
+            const uint64_t sk0 = sbase ^ dtab[c0];
+            const uint64_t sk1 = sbase ^ dtab[c1];
+            const uint32_t d0 = top_digit(sk0, digit_bits);
+            const uint32_t d1 = top_digit(sk1, digit_bits);
+            if (MODE == kMergeCount) {
+              atomicAdd(&lcount[d0], 1u);
+              if (two) atomicAdd(&lcount[d1], 1u);
+            } else {
+              // Run offsets are read before the ranks are taken.
+              const uint64_t r0 = gpos[d0];
+              const uint64_t r1 = gpos[d1];
+              const uint32_t k0 = atomicAdd(&lcount[d0], 1u);
+              uint32_t k1 = 0;
+              if (two) k1 = atomicAdd(&lcount[d1], 1u);
+              emit(sk0, r0, k0);
+              if (two) emit(sk1, r1, k1);
             }
           }
         }
@@ -1505,7 +1519,7 @@ __global__ __launch_bounds__(BT, (BT == kScatterBlock && L <= 2 && T == 2048 && 
     if (threadIdx.x == 0 && wr) atomicAdd((unsigned long long*)&counters[kWritten], (unsigned long long)wr);
   }
   if (MODE != kMergeCount) {
-    const uint64_t u = block_sum_u64<BT / 64>(uniq, scratch);
+    const uint64_t u = block_sum_u64<BT / 64>((uint64_t)uniq, scratch);
     if (threadIdx.x == 0 && u) atomicAdd((unsigned long long*)&counters[kOutStates], (unsigned long long)u);
     if constexpr (P) {
       // pr.lanes is uniform, so every thread takes the same block sums.
