@@ -17,7 +17,9 @@ Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 18]
 The plain form (the default) also streams a bounded mixed-N request set
 (300 requests, N uniform in [10, 18], 0..2 placed rows, saturating arrivals)
 through the request server after the timed region and adds its p50/p99
-latency, requests/s and the ratio to back-to-back plain counts as extra keys.
+latency, requests/s and the ratio to back-to-back plain counts as extra keys;
+then a second stream of 2000 small requests (N uniform in [10, 16]), whose
+figures carry the suffix _small.
 
 The serving form streams requests (N, random non-attacking prefix of 0..2
 rows) with Poisson arrivals at each rate through the request server and
@@ -369,6 +371,9 @@ def main():
     ap.add_argument("--no-serving", action="store_true",
                     help="skip the mixed-N serving leg of the plain form")
     ap.add_argument("--serve-requests", type=int, default=300)
+    ap.add_argument("--serve-small-requests", type=int, default=2000,
+                    help="plain form: a second serving leg of small requests (0: off)")
+    ap.add_argument("--serve-small-n-max", type=int, default=16)
     ap.add_argument("--serve-leg-n-max", type=int, default=18)
     ap.add_argument("--serve-rate", type=float, default=1e5,
                     help="offered request rate of the serving leg (default: saturating)")
@@ -488,6 +493,12 @@ def main():
         serving = serving_leg(pm, args.serve_requests, args.serve_n_min, args.serve_leg_n_max, args.seed,
                               args.serve_rate)
         log(f"serving leg: {json.dumps(serving)}")
+        if args.serve_small_requests:
+            # Requests that are small against the GPU (N <= 16): the regime in
+            # which continuous batching overlaps many requests per launch.
+            serving["small_requests"] = serving_leg(pm, args.serve_small_requests, args.serve_n_min,
+                                                    args.serve_small_n_max, args.seed, args.serve_rate)
+            log(f"serving leg (small requests): {json.dumps(serving['small_requests'])}")
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -534,6 +545,10 @@ def main():
                     "p50_s_paced": serving["paced"]["p50_s"], "p99_s_paced": serving["paced"]["p99_s"],
                     "requests_per_s": serving["requests_per_s"],
                     "vs_back_to_back": serving["vs_back_to_back"], "serving": serving})
+        if "small_requests" in serving:
+            sm = serving["small_requests"]
+            out.update({"requests_per_s_small": sm["requests_per_s"], "p50_s_small": sm["p50_s"],
+                        "p99_s_small": sm["p99_s"], "vs_back_to_back_small": sm["vs_back_to_back"]})
     if cpu and cpu.get("value"):
         # The CPU path counts the board as its 2-row prefix tasks, whose
         # frontiers do not merge with each other: the whole job is the sum of

@@ -157,7 +157,14 @@ struct peps_server {
       std::string e;
       peps::Engine* eng = peps::Engine::get((int)res.device, &e);
       const double ratio = projection.ratio(n, depth);
-      const double cap = eng ? peps::kAdmitFraction * (double)eng->candidate_capacity((int)res.primes.size(), res.records) : 0.0;
+      const double buffer = eng ? (double)eng->candidate_capacity((int)res.primes.size(), res.records) : 0.0;
+      const double cap = peps::kAdmitFraction * buffer;
+      // The leader's own tasks form one merged frontier, as in its plain
+      // count, which the engine runs up to the whole candidate buffer (its
+      // partition levels use exact offsets where the fixed-capacity margin
+      // is missing); splitting it would stop its tasks from merging.  Other
+      // requests join only within the margin.
+      const double lead_cap = buffer;
       const size_t slices = peps::max_slices(n, res.slice_group);
       const size_t limit = ratio > 0 ? slices : std::min<size_t>(slices, 8);
       std::vector<peps::Task> batch;
@@ -170,9 +177,10 @@ struct peps_server {
         if (&r != lead && r.n == n && r.depth == depth) members.push_back(&r);
       }
       for (Request* r : members) {
+        const double admit = r == lead ? lead_cap : cap;
         while (r->next < r->tasks.size() && batch.size() < limit) {
           const double add = r->proxy[r->next] * ratio;
-          if (!batch.empty() && ratio > 0 && cap > 0 && load + add > cap) break;
+          if (!batch.empty() && ratio > 0 && admit > 0 && load + add > admit) break;
           load += add;
           batch.push_back(r->tasks[r->next]);
           owner.push_back(r);
