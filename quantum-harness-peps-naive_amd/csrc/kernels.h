@@ -237,8 +237,11 @@ hipError_t launch_caps(const uint64_t* U, uint32_t W, uint32_t digit_bits, uint3
 // written to out (capacity `cap` records) at offsets[(1<<digit_bits)][W]; if
 // hist is given, the number written per (digit, workgroup) goes there (the
 // run lengths when the offsets are capped or upper bounds).  capw (or null):
-// workgroup w writes at most capw[w] successors per digit; more set bit 4 of
-// counters[kRetry].
+// workgroup w should write at most capw[w] successors per digit.  A longer
+// run sets bit 4 of counters[kRetry] when the kernel ends; its surplus has
+// been written into the ranges of later runs (never past `cap`), so the
+// caller must repeat the SCATTER with other offsets, which rewrites every
+// run the next level reads.
 // LAST: no expansion.  EMIT (lane records only): the merged states are
 // written to `out` as a frontier (keys[cap] then residues[l * cap + i]),
 // positions counted in counters[kEmitted].  Merged states with (key & final_mask) == final_mask

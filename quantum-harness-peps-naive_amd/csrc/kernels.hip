@@ -1204,8 +1204,9 @@ __global__ __launch_bounds__(BT, (BT == kScatterBlock && L <= 2 && T == 2048 && 
   __shared__ uint32_t tres[LL][T];                            // lane records
   __shared__ uint64_t tw[(P && MODE != kMergeCount) ? T : 1];  // packed: exact weights
   __shared__ uint16_t ulist[T];
-  __shared__ uint32_t lcount[DM];
-  // SCATTER: byte offset in `out` of this workgroup's run of each digit.
+  __shared__ uint32_t lcount[MODE == kMergeScatter ? 1 : DM];  // COUNT: digit histogram
+  // SCATTER: byte cursor in `out` of this workgroup's run of each digit (one
+  // LDS atomic gives a successor its store address).
   __shared__ uint64_t gpos[MODE == kMergeScatter ? DM : 1];
   // successor_delta of every column (index: the column's bit position).
   __shared__ uint64_t dtab[kExpand ? 32 : 1];
@@ -1227,9 +1228,10 @@ __global__ __launch_bounds__(BT, (BT == kScatterBlock && L <= 2 && T == 2048 && 
   }
   if (kExpand) {
     for (uint32_t d = threadIdx.x; d < D; d += BT) {
-      lcount[d] = 0;
-      if (MODE == kMergeScatter)
+      if constexpr (MODE == kMergeScatter)
         gpos[d] = offsets[(uint64_t)d * W + blockIdx.x] * sizeof(Rec<L>);
+      else
+        lcount[d] = 0;
     }
     if (threadIdx.x < 32) dtab[threadIdx.x] = threadIdx.x < g.n ? successor_delta(1u << threadIdx.x, g) : 0;
   }
@@ -1272,11 +1274,13 @@ __global__ __launch_bounds__(BT, (BT == kScatterBlock && L <= 2 && T == 2048 && 
     bucket_range(segs, in_cap, b0, &plo, &phi);
     if (phi - plo <= kCap) load_batch(plo, phi);
   }
-  // Capped SCATTER runs: at most wcap successors per digit (more sets bit 4
-  // of counters[kRetry] and writes nothing past the run; the engine then
-  // repeats the SCATTER with upper-bound offsets).
-  const uint64_t wcap64 = (MODE == kMergeScatter && capw) ? capw[blockIdx.x] : ~0ull;
-  const uint32_t wcap = wcap64 < 0xffffffffull ? (uint32_t)wcap64 : 0xffffffffu;
+  // Capped SCATTER runs: at most wcap successors per digit.  A run that
+  // ends longer sets bit 4 of counters[kRetry] (checked once, after the last
+  // bucket); its surplus stores have then landed in the ranges of later runs
+  // of the same output region, and the engine repeats the whole SCATTER with
+  // upper-bound offsets, which rewrites every position the next level reads.
+  // Every store stays inside the output region.
+  const uint64_t wcap = (MODE == kMergeScatter && capw) ? capw[blockIdx.x] : ~0ull;
   const uint64_t out_end = cap * sizeof(Rec<L>);  // bytes of the output region
 
   for (uint32_t b = b0; b < b1; ++b) {
@@ -1454,9 +1458,9 @@ __global__ __launch_bounds__(BT, (BT == kScatterBlock && L <= 2 && T == 2048 && 
           const Board bs = unpack(key, g);
           uint32_t free = legal_positions(bs, g);
           const uint64_t sbase = successor_base(key, bs, g);
-          // Two successors per step: their table reads, run offsets and rank
-          // atomics are in flight together.
-          auto emit = [&](uint64_t sk, uint64_t run0, uint32_t rank) {
+          // Two successors per step: their table reads and run-cursor atomics
+          // are in flight together.
+          auto emit = [&](uint64_t sk, uint64_t at) {
             Rec<L> rec;
             if constexpr (P) {
               rec = make_packed(sk, wv, g.key_bits);
@@ -1465,12 +1469,10 @@ __global__ __launch_bounds__(BT, (BT == kScatterBlock && L <= 2 && T == 2048 && 
 #pragma unroll
               for (int l = 0; l < L; ++l) rec.r[l] = v[l];
             }
-            const uint64_t at = run0 + (uint64_t)rank * sizeof(Rec<L>);
-            const bool in_run = rank < wcap, in_region = at < out_end;
-            if (in_run && in_region)
+            if (at < out_end)
               *(Rec<L>*)((char*)out + at) = rec;
             else
-              overflow |= in_run ? 2u : 16u;
+              overflow |= 2u;
           };
           while (free) {
             const uint32_t c0 = (uint32_t)__builtin_ctz(free);
@@ -1486,14 +1488,12 @@ __global__ __launch_bounds__(BT, (BT == kScatterBlock && L <= 2 && T == 2048 && 
               atomicAdd(&lcount[d0], 1u);
               if (two) atomicAdd(&lcount[d1], 1u);
             } else {
-              // Run offsets are read before the ranks are taken.
-              const uint64_t r0 = gpos[d0];
-              const uint64_t r1 = gpos[d1];
-              const uint32_t k0 = atomicAdd(&lcount[d0], 1u);
-              uint32_t k1 = 0;
-              if (two) k1 = atomicAdd(&lcount[d1], 1u);
-              emit(sk0, r0, k0);
-              if (two) emit(sk1, r1, k1);
+              constexpr unsigned long long kStep = sizeof(Rec<L>);
+              const uint64_t a0 = atomicAdd((unsigned long long*)&gpos[d0], kStep);
+              uint64_t a1 = 0;
+              if (two) a1 = atomicAdd((unsigned long long*)&gpos[d1], kStep);
+              emit(sk0, a0);
+              if (two) emit(sk1, a1);
             }
           }
         }
@@ -1507,14 +1507,23 @@ __global__ __launch_bounds__(BT, (BT == kScatterBlock && L <= 2 && T == 2048 && 
   }
 
   // ---- flush statistics
-  if (MODE == kMergeCount || (MODE == kMergeScatter && hist)) {
-    // COUNT: the digit histogram.  SCATTER with upper-bound offsets: the
-    // length of every run it wrote (the rest of each run's range is a gap).
+  if constexpr (MODE == kMergeCount) {
+    // the digit histogram
     for (uint32_t d = threadIdx.x; d < D; d += BT) hist[(uint64_t)d * W + blockIdx.x] = lcount[d];
   }
-  if (MODE == kMergeScatter) {
+  if constexpr (MODE == kMergeScatter) {
+    // Run lengths from the cursors; with upper-bound offsets (hist set) the
+    // length of every run is written out (the rest of its range is a gap).
     uint64_t wr = 0;
-    for (uint32_t d = threadIdx.x; d < D; d += BT) wr += lcount[d];
+    for (uint32_t d = threadIdx.x; d < D; d += BT) {
+      const uint64_t len = gpos[d] / sizeof(Rec<L>) - offsets[(uint64_t)d * W + blockIdx.x];
+      if (hist) hist[(uint64_t)d * W + blockIdx.x] = (uint32_t)len;
+      if (len > wcap) overflow |= 16u;
+      wr += len;
+    }
+    // A store past the region: with capped runs, the surplus of an
+    // overflowing run (a retry); with exact or upper-bound offsets, an error.
+    if (capw && (overflow & 2u)) overflow = (overflow & ~2u) | 16u;
     wr = block_sum_u64<BT / 64>(wr, scratch);
     if (threadIdx.x == 0 && wr) atomicAdd((unsigned long long*)&counters[kWritten], (unsigned long long)wr);
   }
