@@ -129,6 +129,24 @@ def test_edge_requests():
 
 
 @pytest.mark.gpu
+def test_lone_request_runs_as_one_merged_launch():
+    # A request alone in its class fills its launch up to the whole candidate
+    # buffer: once the class is calibrated (the first launch of a class holds
+    # at most 8 tasks), all of its prefix tasks run in one launch, merged as in
+    # its plain count, and the count equals known_count.
+    with pm.Server() as s:
+        first = s.wait(s.submit(16, []), 300)
+        again = [s.wait(s.submit(16, []), 300) for _ in range(2)]
+    for r in [first] + again:
+        show("lone N=16", r.tasks, r.launches, r.count)
+        assert r.status == 0, r.error
+        assert r.count == o.known_count(16)
+    assert first.launches >= 1
+    for r in again:
+        assert r.tasks > 8 and r.launches == 1, (r.tasks, r.launches)
+
+
+@pytest.mark.gpu
 def test_submit_errors_and_queue_limit():
     with pm.Server(max_pending=2) as s:
         with pytest.raises(pm.PepsError, match="N <= 21"):
