@@ -162,9 +162,13 @@ struct peps_server {
       // The leader's own tasks form one merged frontier, as in its plain
       // count, which the engine runs up to the whole candidate buffer (its
       // partition levels use exact offsets where the fixed-capacity margin
-      // is missing); splitting it would stop its tasks from merging.  Other
-      // requests join only within the margin.
-      const double lead_cap = buffer;
+      // is missing).  When all of its remaining tasks are projected to fit
+      // kLeadFraction of the buffer they go into this launch together, since
+      // splitting them would stop them merging; a request too large for one
+      // launch is packed within the margin, like the requests that join it.
+      constexpr double kLeadFraction = 0.8;
+      const double lead_rest = ratio > 0 ? lead->remaining * ratio : 0.0;
+      const double lead_cap = ratio > 0 && lead_rest <= kLeadFraction * buffer ? buffer : cap;
       const size_t slices = peps::max_slices(n, res.slice_group);
       const size_t limit = ratio > 0 ? slices : std::min<size_t>(slices, 8);
       std::vector<peps::Task> batch;
