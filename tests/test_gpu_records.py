@@ -115,6 +115,21 @@ def test_prefix_counts_packed_match_dfs():
         assert pm.count_prefix(12, cols, opts(PEPS_RECORDS_PACKED)) == o.count_prefix_dfs(12, cols), cols
 
 
+def test_n21_prefixes_at_the_widest_key():
+    # N = 21 is the widest board of the u64 key (3N = 63 bits, no slice
+    # bits), where the packed weight field is 96 - 63 = 33 bits.  Three-row
+    # prefixes (18 rows of work): packed (default) and lane records against
+    # the oracle's parallel prefix DFS, and the mirror prefix (c -> 20 - c),
+    # whose count the board's reflection fixes.
+    for cols in ([0, 2, 4], [10, 3, 17]):
+        show("n21 prefix", cols)
+        want = o.count_prefix_parallel(21, cols, threads=16)
+        got = pm.count_prefix(21, cols)
+        lanes = pm.count_prefix(21, cols, opts(PEPS_RECORDS_LANES))
+        mirror = pm.count_prefix(21, [20 - c for c in cols])
+        assert got == lanes == mirror == want, (cols, got, lanes, mirror, want)
+
+
 def test_published_crt_release_n18_n19():
     # benchmarks/e45_wide_crt_release.csv rows for N = 18 and 19: counts and
     # both CRT residues.  N = 18 through the default (packed) plain count;
