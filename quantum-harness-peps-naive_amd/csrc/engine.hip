@@ -958,7 +958,7 @@ int Engine::pipeline(const RunSpec& spec, Plan* plan, bool eager, RunOut* out, s
       const uint64_t nseg = bk.nseg;
       const uint64_t nout = nseg << dk;
       const bool last_level = li + 1 == pd.size();
-      uint32_t chunks = partition_chunks(Crow, nseg);
+      uint32_t chunks = partition_chunks(Crow, nseg, rn != nullptr);
       if (rn) chunks = std::max<uint32_t>(chunks, (rn->per_seg + kMaxChunkRuns - 1) / kMaxChunkRuns);
       // Twice the records where the arena allows, so that fixed-capacity
       // buckets keep their margin.

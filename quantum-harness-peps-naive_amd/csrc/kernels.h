@@ -176,9 +176,14 @@ struct Buckets {
 //    bucket that would pass out_cap sets bit 5 of counters[kError] and
 //    writes nothing beyond it (the engine repeats the level exactly).
 // A write at or beyond `cap` (the output region's records) sets bit 1.
-constexpr uint64_t kPartChunk = 65536;     // target records per partition workgroup
+// Target records per partition work item: contiguous segments (a row's last
+// level) take 64 Ki records; levels that read SCATTER runs take 1 Mi, since
+// longer items append longer stretches to each output bucket (at least one
+// item per kMaxChunkRuns runs keeps small rows spread over the GPU).
+constexpr uint64_t kPartChunk = 65536;
+constexpr uint64_t kPartChunkRuns = 1048576;
 constexpr uint64_t kMaxPartChunks = 4096;
-uint32_t partition_chunks(uint64_t records, uint64_t nseg);
+uint32_t partition_chunks(uint64_t records, uint64_t nseg, bool runs = false);
 // Run mode of a partition level (its input is the SCATTER output with
 // capped or upper-bound offsets): segment s is the runs w < per_seg, run
 // (s, w) being the records [start[s * per_seg + w], start[...] +

@@ -1637,8 +1637,8 @@ hipError_t launch_scan(const uint32_t* hist, uint32_t D, uint32_t W, uint32_t gr
   return hipGetLastError();
 }
 
-uint32_t partition_chunks(uint64_t records, uint64_t nseg) {
-  const uint64_t per = nseg * kPartChunk;
+uint32_t partition_chunks(uint64_t records, uint64_t nseg, bool runs) {
+  const uint64_t per = nseg * (runs ? kPartChunkRuns : kPartChunk);
   const uint64_t c = (records + per - 1) / per;
   return (uint32_t)(c < 1 ? 1 : (c > kMaxPartChunks ? kMaxPartChunks : c));
 }
