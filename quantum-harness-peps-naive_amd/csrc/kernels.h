@@ -177,10 +177,12 @@ struct Buckets {
 //    writes nothing beyond it (the engine repeats the level exactly).
 // A write at or beyond `cap` (the output region's records) sets bit 1.
 // Target records per partition work item: contiguous segments (a row's last
-// level) take 64 Ki records; levels that read SCATTER runs take 1 Mi, since
-// longer items append longer stretches to each output bucket (at least one
-// item per kMaxChunkRuns runs keeps small rows spread over the GPU).
-constexpr uint64_t kPartChunk = 65536;
+// level) take 48 Ki records (at the peak rows a segment's items then fill
+// whole LDS stages but for a nearly full last one); levels that read SCATTER
+// runs take 1 Mi, since longer items append longer stretches to each output
+// bucket (at least one item per kMaxChunkRuns runs keeps small rows spread
+// over the GPU).
+constexpr uint64_t kPartChunk = 49152;
 constexpr uint64_t kPartChunkRuns = 1048576;
 constexpr uint64_t kMaxPartChunks = 4096;
 uint32_t partition_chunks(uint64_t records, uint64_t nseg, bool runs = false);
