@@ -176,10 +176,10 @@ struct Buckets {
 //    bucket that would pass out_cap sets bit 5 of counters[kError] and
 //    writes nothing beyond it (the engine repeats the level exactly).
 // A write at or beyond `cap` (the output region's records) sets bit 1.
-// Target records per partition work item: contiguous segments (a row's last
-// level) take 48 Ki records (at the peak rows a segment's items then fill
-// whole LDS stages but for a nearly full last one); levels that read SCATTER
-// runs take 1 Mi, since longer items append longer stretches to each output
+// Target records per partition work item, the fastest of a sweep at N = 18:
+// contiguous segments (a row's last level) take 48 Ki records (24-96 Ki
+// measured); levels that read SCATTER runs take 1 Mi (64 Ki - 1 Mi
+// measured), since longer items append longer stretches to each output
 // bucket (at least one item per kMaxChunkRuns runs keeps small rows spread
 // over the GPU).
 constexpr uint64_t kPartChunk = 49152;
